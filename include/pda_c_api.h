@@ -1,0 +1,154 @@
+/*
+ * pda_c_api.h -- C ABI of the MI355X-native pressio-demoapps engine.
+ *
+ * This is the drop-in boundary for the reference's problem API
+ * (include/pressiodemoapps/adapter_cpp.hpp:60-264 and the per-family
+ * create_problem_eigen factories).  Every entry point takes plain pointers
+ * and sizes; state, velocity and Jacobian values live in HIP device memory.
+ * Streams are passed as void* and are hipStream_t values (NULL = default).
+ *
+ * Conventions:
+ *   - return 0 on success, a negative PDA_E* code on failure;
+ *     pda_last_error() returns a thread-local message for the last failure;
+ *   - a problem handle holds a reference on its mesh;
+ *   - one problem handle is not re-entrant (as the reference, whose
+ *     evaluation methods mutate internal ghost buffers); calls are ordered on
+ *     the stream they are given; distinct handles are independent.
+ */
+#ifndef PDA_C_API_H
+#define PDA_C_API_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pda_mesh_s *pda_mesh;
+typedef struct pda_problem_s *pda_problem;
+typedef void *pda_stream; /* hipStream_t */
+
+enum {
+  PDA_OK = 0,
+  PDA_E_INVALID_ARG = -1,
+  PDA_E_UNSUPPORTED = -2,
+  PDA_E_IO = -3,
+  PDA_E_HIP = -4,
+  PDA_E_STATE = -5
+};
+
+/* families: one per reference problem header */
+enum {
+  PDA_EULER1D = 1,    /* include/pressiodemoapps/euler1d.hpp:65-70  */
+  PDA_EULER2D = 2,    /* include/pressiodemoapps/euler2d.hpp:67-77  */
+  PDA_EULER3D = 3,    /* include/pressiodemoapps/euler3d.hpp:65-68  */
+  PDA_SWE2D = 4,      /* include/pressiodemoapps/swe2d.hpp:66-69    */
+  PDA_BURGERS2D = 5,  /* include/pressiodemoapps/advection_diffusion2d.hpp:65-68 */
+  PDA_ADR2D = 6       /* include/pressiodemoapps/advection_diffusion_reaction2d.hpp:61-62 */
+};
+
+/* InviscidFluxReconstruction, include/pressiodemoapps/schemes_info.hpp:61-63 */
+enum { PDA_FIRSTORDER = 0, PDA_WENO3 = 1, PDA_WENO5 = 2 };
+
+const char *pda_last_error(void);
+const char *pda_version(void);
+
+/* ---------------------------------------------------------------- mesh */
+
+/* Reads a mesh directory in the reference text format (info.dat,
+   coordinates.dat, connectivity.dat).  Replaces
+   load_cellcentered_uniform_mesh_eigen, include/pressiodemoapps/mesh.hpp:87-91
+   and impl/mesh_ccu.hpp:359-448; returns PDA_E_IO instead of exit(). */
+int pda_mesh_load(const char *dir, pda_mesh *out);
+
+/* Builds the natural-order structured mesh that
+   meshing_scripts/create_full_mesh.py:32-213 writes (GID = gk*Nx*Ny+gj*Nx+gi,
+   neighbour columns [l0 f0 r0 b0 (d0 u0)] per layer, periodic wrap per axis),
+   including its %.14f round trip of dx and of the cell centres.
+   n[dim], bounds = {xmin,xmax,ymin,ymax,zmin,zmax}; periodic bit a = axis a. */
+int pda_mesh_create_structured(int dim, const int64_t *n, const double *bounds,
+                               int stencil, int periodic_mask, pda_mesh *out);
+
+/* Builds a sample mesh from a full mesh and sample-cell ids, as
+   meshing_scripts/create_sample_mesh.py:28-137 does.  stencil_gids_out
+   (nullable, size >= returned stencil size) receives the stencil-mesh cells
+   as full-mesh ids, sample_gids_out (nullable) the sorted unique sample
+   cells. */
+int pda_mesh_create_sample(pda_mesh full, const int32_t *sample_gids, int64_t n,
+                           pda_mesh *out);
+int pda_mesh_sample_gids(pda_mesh sample, int32_t *sample_gids_out,
+                         int32_t *stencil_gids_out);
+
+/* Writes info.dat / coordinates.dat / connectivity.dat in the reference
+   format (meshing_scripts/create_full_mesh.py:152-213). */
+int pda_mesh_write(pda_mesh mesh, const char *dir);
+
+int pda_mesh_info(pda_mesh mesh, int *dim, int *stencil, int64_t *n_sample,
+                  int64_t *n_stencil, double *d /*[3]*/);
+/* host views valid for the mesh lifetime */
+const int32_t *pda_mesh_graph(pda_mesh mesh);      /* n_sample x ncols */
+const double *pda_mesh_coords(pda_mesh mesh, int axis); /* n_stencil */
+/* counts of inner / near-boundary graph rows, impl/mesh_ccu.hpp:385-447 */
+int pda_mesh_row_classes(pda_mesh mesh, int64_t *n_inner, int64_t *n_near);
+void pda_mesh_destroy(pda_mesh mesh);
+
+/* ------------------------------------------------------------- problem */
+
+/* Replaces create_problem_eigen(mesh, ProblemEnum, InviscidFluxReconstruction
+   [, icFlag][, parameter map]) for the six families (e.g.
+   include/pressiodemoapps/euler2d.hpp:88-173, swe2d.hpp:143-195,
+   advection_diffusion2d.hpp:80-179, advection_diffusion_reaction2d.hpp:111-180).
+   ic_flag <= 0 selects the family default.  Parameter names are the
+   reference's ("gamma", "gravity", "coriolis", "pulseMagnitude",
+   "diffusion", "riemannTopRightPressure", ...).  Unknown names and
+   unsupported problem enums return PDA_E_INVALID_ARG / PDA_E_UNSUPPORTED. */
+int pda_problem_create(pda_mesh mesh, int family, int prob, int scheme,
+                       int ic_flag, const char *const *keys, const double *vals,
+                       int nparams, pda_problem *out);
+void pda_problem_destroy(pda_problem p);
+
+/* numDofPerCell / totalDofSampleMesh / totalDofStencilMesh and nonZeros of
+   the Jacobian, adapter_cpp.hpp:93-103 */
+int pda_problem_sizes(pda_problem p, int *ndpc, int64_t *n_sample_dofs,
+                      int64_t *n_stencil_dofs, int64_t *nnz);
+/* parameter lookup, e.g. gamma() / gravity() / coriolis() / queryParameter */
+int pda_problem_parameter(pda_problem p, const char *name, double *value);
+
+/* initialCondition(), into device memory of n_stencil_dofs doubles */
+int pda_initial_condition(pda_problem p, double *d_U, pda_stream stream);
+
+/* Jacobian sparsity, fixed at creation: scalar CSR with the reference's row
+   order and sorted columns (Eigen setFromTriplets + makeCompressed,
+   impl/euler_2d_prob_class.hpp:223-237).  Device pointers, valid for the
+   problem lifetime.  The host variant copies into caller arrays of
+   n_sample_dofs+1 and nnz entries. */
+int pda_jacobian_pattern(pda_problem p, int64_t *n_rows, int64_t *nnz,
+                         const int64_t **d_rowptr, const int32_t **d_colidx);
+int pda_jacobian_pattern_host(pda_problem p, int64_t *rowptr, int32_t *colidx);
+
+/* rightHandSide(U, t, V): adapter_cpp.hpp:193-199 */
+int pda_velocity(pda_problem p, const double *d_U, double t, double *d_V,
+                 pda_stream stream);
+
+/* rightHandSideAndJacobian(U, t, V, J) (adapter_cpp.hpp:201-208) when d_V is
+   non-NULL, jacobian(U, t, J) (adapter_cpp.hpp:223-229) when it is NULL.
+   d_Jvals has nnz entries in the order of pda_jacobian_pattern. */
+int pda_jacobian(pda_problem p, const double *d_U, double t, double *d_Jvals,
+                 double *d_V, pda_stream stream);
+
+/* applyJacobian(U, B, t, AJ): adapter_cpp.hpp:231-265.  B has n_stencil_dofs
+   rows and ncols columns; layout 0 = column-major, 1 = row-major; AJ has
+   n_sample_dofs rows in the same layout. */
+int pda_apply_jacobian(pda_problem p, const double *d_U, double t,
+                       const double *d_B, int64_t ncols, int layout,
+                       double *d_AJ, pda_stream stream);
+
+/* Block view of the Jacobian (m x m blocks).  For each block-row r and block
+   slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values
+   start at d_Jvals[rowptr[r*m] + s*m] with row stride nblk[r]*m. */
+int pda_jacobian_blocks(pda_problem p, int *maxb, const int32_t **d_nblk,
+                        const int32_t **d_bcol);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

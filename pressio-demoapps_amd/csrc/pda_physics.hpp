@@ -1,0 +1,452 @@
+// pda_physics.hpp -- device-side numerics of the finite-volume hot path:
+// WENO3/WENO5 reconstruction (values and stencil gradients) and the face
+// fluxes with their exact Jacobians for every family.
+//
+// Formulas follow the reference files cited per function; evaluation order
+// is kept close to them, and FP64 throughout.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pda {
+namespace phys {
+
+constexpr double kEs = 1.e-30;
+
+// ---------------------------------------------------------------- WENO3
+// impl/weno3.hpp:56-132.  q = {q_{i-1}, q_i, q_{i+1}, q_{i+2}}
+__device__ __forceinline__ void weno3(const double *q, double &un, double &up) {
+  constexpr double eps = 1.e-6, c13 = 1. / 3., c23 = 2. / 3.;
+  {
+    const double p0 = (-q[0] + 3. * q[1]) * 0.5;
+    const double p1 = (q[1] + q[2]) * 0.5;
+    const double B0 = (q[0] - q[1]) * (q[0] - q[1]);
+    const double B1 = (q[1] - q[2]) * (q[1] - q[2]);
+    const double a0 = c13 / (eps * eps + 2. * eps * B0 + B0 * B0);
+    const double a1 = c23 / (eps * eps + 2. * eps * B1 + B1 * B1);
+    const double s = 1. / (a0 + a1);
+    un = (a0 * s) * p0 + (a1 * s) * p1;
+  }
+  {
+    const double p0 = (q[1] + q[2]) * 0.5;
+    const double p1 = (3. * q[2] - q[3]) * 0.5;
+    const double B0 = (q[1] - q[2]) * (q[1] - q[2]);
+    const double B1 = (q[2] - q[3]) * (q[2] - q[3]);
+    const double a0 = c23 / (eps * eps + 2. * eps * B0 + B0 * B0);
+    const double a1 = c13 / (eps * eps + 2. * eps * B1 + B1 * B1);
+    const double s = 1. / (a0 + a1);
+    up = (a0 * s) * p0 + (a1 * s) * p1;
+  }
+}
+
+// impl/weno3.hpp:134-246: values plus d(face)/d(q_j), j = 0..3
+__device__ __forceinline__ void weno3_grad(const double *q, double &un, double &up,
+                                           double *gn, double *gp) {
+  constexpr double eps = 1.e-6, c13 = 1. / 3., c23 = 2. / 3.;
+  {
+    const double qim1 = q[0], qi = q[1], qip1 = q[2];
+    const double p0 = (-qim1 + 3. * qi) * 0.5, p1 = (qi + qip1) * 0.5;
+    const double d0 = qim1 - qi, d1 = qi - qip1;
+    const double B0 = d0 * d0, B1 = d1 * d1;
+    const double a0 = c13 / (eps * eps + 2. * eps * B0 + B0 * B0);
+    const double a1 = c23 / (eps * eps + 2. * eps * B1 + B1 * B1);
+    const double s = 1. / (a0 + a1);
+    const double e0 = B0 + eps, e1 = B1 + eps;
+    const double e03 = e0 * e0 * e0, e13 = e1 * e1 * e1;
+    const double S = 2. / (3. * (e1 * e1)) + 1. / (3. * (e0 * e0));
+    const double S2 = S * S;
+    const double da0[3] = {-4. * d0 / (3. * e03), 4. * d0 / (3. * e03), 0.};
+    const double da1[3] = {0., -8. * d1 / (3. * e13), 8. * d1 / (3. * e13)};
+    const double ds[3] = {(4. * d0) / (3. * e03 * S2),
+                          -((4. * d0) / (3. * e03) - (8. * d1) / (3. * e13)) / S2,
+                          -(8. * d1) / (3. * S2 * e13)};
+    const double dp0[3] = {-0.5, 1.5, 0.}, dp1[3] = {0., 0.5, 0.5};
+    const double w0 = a0 * s, w1 = a1 * s;
+    for (int i = 0; i < 3; ++i) {
+      double v = (da0[i] * s + ds[i] * a0) * p0 + dp0[i] * w0;
+      v += (da1[i] * s + ds[i] * a1) * p1 + dp1[i] * w1;
+      gn[i] = v;
+    }
+    gn[3] = 0.;
+    un = w0 * p0 + w1 * p1;
+  }
+  {
+    const double qi = q[1], qip1 = q[2], qip2 = q[3];
+    const double p0 = (qi + qip1) * 0.5, p1 = (3. * qip1 - qip2) * 0.5;
+    const double d0 = qi - qip1, d1 = qip1 - qip2;
+    const double B0 = d0 * d0, B1 = d1 * d1;
+    const double a0 = c23 / (eps * eps + 2. * eps * B0 + B0 * B0);
+    const double a1 = c13 / (eps * eps + 2. * eps * B1 + B1 * B1);
+    const double s = 1. / (a0 + a1);
+    const double w0 = a0 * s, w1 = a1 * s;
+    const double e0 = B0 + eps, e1 = B1 + eps;
+    const double e03 = e0 * e0 * e0, e13 = e1 * e1 * e1;
+    const double S = 1. / (3. * (e1 * e1)) + 2. / (3. * (e0 * e0));
+    const double S2 = S * S;
+    const double da0[3] = {-8. * d0 / (3. * e03), 8. * d0 / (3. * e03), 0.};
+    const double da1[3] = {0., -4. * d1 / (3. * e13), 4. * d1 / (3. * e13)};
+    const double ds[3] = {8. * d0 / (3. * e03 * S2),
+                          -(8. * d0 / (3. * e03) - 4. * d1 / (3. * e13)) / S2,
+                          -4. * d1 / (3. * S2 * e13)};
+    const double dp0[3] = {0.5, 0.5, 0.}, dp1[3] = {0., 1.5, -0.5};
+    for (int i = 0; i < 3; ++i) {
+      double v = (da0[i] * s + ds[i] * a0) * p0 + dp0[i] * w0;
+      v += (da1[i] * s + ds[i] * a1) * p1 + dp1[i] * w1;
+      gp[i + 1] = v;
+    }
+    gp[0] = 0.;
+    up = w0 * p0 + w1 * p1;
+  }
+}
+
+// ---------------------------------------------------------------- WENO5
+// impl/weno5.hpp:56-178.  q = {q_{i-2}, ..., q_{i+3}}
+__device__ __forceinline__ void weno5(const double *q, double &un, double &up) {
+  constexpr double eps = 1.e-6, c1312 = 13. / 12., c16 = 1. / 6.;
+  {
+    const double p0 = (2. * q[0] - 7. * q[1] + 11. * q[2]) * c16;
+    const double p1 = (-q[1] + 5. * q[2] + 2. * q[3]) * c16;
+    const double p2 = (2. * q[2] + 5. * q[3] - q[4]) * c16;
+    const double t0 = q[0] - 2. * q[1] + q[2], s0 = q[0] - 4. * q[1] + 3. * q[2];
+    const double t1 = q[1] - 2. * q[2] + q[3], s1 = q[1] - q[3];
+    const double t2 = q[2] - 2. * q[3] + q[4], s2 = 3. * q[2] - 4. * q[3] + q[4];
+    const double B0 = c1312 * t0 * t0 + 0.25 * s0 * s0;
+    const double B1 = c1312 * t1 * t1 + 0.25 * s1 * s1;
+    const double B2 = c1312 * t2 * t2 + 0.25 * s2 * s2;
+    const double a0 = 0.1 / (eps * eps + 2. * eps * B0 + B0 * B0);
+    const double a1 = 0.6 / (eps * eps + 2. * eps * B1 + B1 * B1);
+    const double a2 = 0.3 / (eps * eps + 2. * eps * B2 + B2 * B2);
+    const double s = 1. / (a0 + a1 + a2);
+    un = (a0 * s) * p0 + (a1 * s) * p1 + (a2 * s) * p2;
+  }
+  {
+    const double p0 = (-q[1] + 5. * q[2] + 2. * q[3]) * c16;
+    const double p1 = (2. * q[2] + 5. * q[3] - q[4]) * c16;
+    const double p2 = (11. * q[3] - 7. * q[4] + 2. * q[5]) * c16;
+    const double t0 = q[1] - 2. * q[2] + q[3], s0 = q[1] - 4. * q[2] + 3. * q[3];
+    const double t1 = q[2] - 2. * q[3] + q[4], s1 = q[2] - q[4];
+    const double t2 = q[3] - 2. * q[4] + q[5], s2 = 3. * q[3] - 4. * q[4] + q[5];
+    const double B0 = c1312 * t0 * t0 + 0.25 * s0 * s0;
+    const double B1 = c1312 * t1 * t1 + 0.25 * s1 * s1;
+    const double B2 = c1312 * t2 * t2 + 0.25 * s2 * s2;
+    const double a0 = 0.3 / (eps * eps + 2. * eps * B0 + B0 * B0);
+    const double a1 = 0.6 / (eps * eps + 2. * eps * B1 + B1 * B1);
+    const double a2 = 0.1 / (eps * eps + 2. * eps * B2 + B2 * B2);
+    const double s = 1. / (a0 + a1 + a2);
+    up = (a0 * s) * p0 + (a1 * s) * p1 + (a2 * s) * p2;
+  }
+}
+
+// one side of impl/weno5.hpp:180-434.  c = linear weights {d0,d1,d2};
+// P[k][i] = d p_k / d q_i; T/S = the two smoothness building blocks with
+// their gradients; returns value, writes grad[0..4]
+__device__ __forceinline__ double weno5_side_grad(const double p[3], const double B[3],
+                                                  const double dB[3][5], const double dp[3][5],
+                                                  const double c[3], double *grad) {
+  constexpr double eps = 1.e-6;
+  double a[3], e3[3];
+  for (int k = 0; k < 3; ++k) {
+    const double e = eps + B[k];
+    a[k] = c[k] / (e * e);
+    e3[k] = e * e * e;
+  }
+  const double s = 1. / (a[0] + a[1] + a[2]);
+  const double S = a[2] + a[1] + a[0];
+  const double mS2 = -1. / (S * S);
+  const double w0 = a[0] * s, w1 = a[1] * s, w2 = a[2] * s;
+  for (int i = 0; i < 5; ++i) {
+    const double da0 = -2. * c[0] / e3[0] * dB[0][i];
+    const double da1 = -2. * c[1] / e3[1] * dB[1][i];
+    const double da2 = -2. * c[2] / e3[2] * dB[2][i];
+    const double ds = mS2 * (da0 + da1 + da2);
+    double v = (da0 * s + ds * a[0]) * p[0] + dp[0][i] * w0;
+    v += (da1 * s + ds * a[1]) * p[1] + dp[1][i] * w1;
+    v += (da2 * s + ds * a[2]) * p[2] + dp[2][i] * w2;
+    grad[i] = v;
+  }
+  return w0 * p[0] + w1 * p[1] + w2 * p[2];
+}
+
+// values plus d(face)/d(q_j), j = 0..5 (gn[5] = gp[0] = 0)
+__device__ __forceinline__ void weno5_grad(const double *q, double &un, double &up,
+                                           double *gn, double *gp) {
+  constexpr double c1312 = 13. / 12., c16 = 1. / 6.;
+  {
+    const double qim2 = q[0], qim1 = q[1], qi = q[2], qip1 = q[3], qip2 = q[4];
+    const double p[3] = {(2. * qim2 - 7. * qim1 + 11. * qi) * c16,
+                         (-qim1 + 5. * qi + 2. * qip1) * c16,
+                         (2. * qi + 5. * qip1 - qip2) * c16};
+    const double t0 = qim2 - 2. * qim1 + qi, s0 = qim2 - 4. * qim1 + 3. * qi;
+    const double t1 = qim1 - 2. * qi + qip1, s1 = qim1 - qip1;
+    const double t2 = qi - 2. * qip1 + qip2, s2 = 3. * qi - 4. * qip1 + qip2;
+    const double B[3] = {c1312 * (t0 * t0) + 0.25 * (s0 * s0),
+                         c1312 * (t1 * t1) + 0.25 * (s1 * s1),
+                         c1312 * (t2 * t2) + 0.25 * (s2 * s2)};
+    const double dB[3][5] = {
+        {13. * t0 / 6. + s0 / 2., -13. * t0 / 3. - 2. * s0, 13. * t0 / 6. + 3. * s0 / 2., 0., 0.},
+        {0., 13. * t1 / 6. + s1 / 2., -13. * t1 / 3., 13. * t1 / 6. - s1 / 2., 0.},
+        {0., 0., 13. * t2 / 6. + 3. * s2 / 2., -13. * t2 / 3. - 2. * s2, 13. * t2 / 6. + s2 / 2.}};
+    const double dp[3][5] = {{1. / 3., -7. / 6., 11. / 6., 0., 0.},
+                             {0., -1. / 6., 5. / 6., 1. / 3., 0.},
+                             {0., 0., 1. / 3., 5. / 6., -1. / 6.}};
+    const double c[3] = {0.1, 0.6, 0.3};
+    un = weno5_side_grad(p, B, dB, dp, c, gn);
+    gn[5] = 0.;
+  }
+  {
+    const double qim1 = q[1], qi = q[2], qip1 = q[3], qip2 = q[4], qip3 = q[5];
+    const double p[3] = {(-qim1 + 5. * qi + 2. * qip1) * c16,
+                         (2. * qi + 5. * qip1 - qip2) * c16,
+                         (11. * qip1 - 7. * qip2 + 2. * qip3) * c16};
+    const double t0 = qim1 - 2. * qi + qip1, s0 = qim1 - 4. * qi + 3. * qip1;
+    const double t1 = qi - 2. * qip1 + qip2, s1 = qi - qip2;
+    const double t2 = qip1 - 2. * qip2 + qip3, s2 = 3. * qip1 - 4. * qip2 + qip3;
+    const double B[3] = {c1312 * (t0 * t0) + 0.25 * (s0 * s0),
+                         c1312 * (t1 * t1) + 0.25 * (s1 * s1),
+                         c1312 * (t2 * t2) + 0.25 * (s2 * s2)};
+    const double dB[3][5] = {
+        {s0 / 2. + 13. * t0 / 6., -2. * s0 - 13. * t0 / 3., 3. * s0 / 2. + 13. * t0 / 6., 0., 0.},
+        {0., 13. * t1 / 6. + s1 / 2., -13. * t1 / 3., 13. * t1 / 6. - s1 / 2., 0.},
+        {0., 0., 13. * t2 / 6. + 3. * s2 / 2., -13. * t2 / 3. - 2. * s2, 13. * t2 / 6. + s2 / 2.}};
+    const double dp[3][5] = {{-1. / 6., 5. / 6., 1. / 3., 0., 0.},
+                             {0., 1. / 3., 5. / 6., -1. / 6., 0.},
+                             {0., 0., 11. / 6., -7. / 6., 1. / 3.}};
+    const double c[3] = {0.3, 0.6, 0.1};
+    up = weno5_side_grad(p, B, dB, dp, c, gp + 1);
+    gp[0] = 0.;
+  }
+}
+
+// ------------------------------------------------------------ Euler
+// Rusanov flux for 1, 2, 3 dimensions (M = DIM+2 conserved variables),
+// impl/euler_rusanov_flux_values_function.hpp:54-208.  A = face normal axis.
+template <int DIM, int A>
+__device__ __forceinline__ void euler_flux(const double *qL, const double *qR, double g,
+                                           double *F) {
+  constexpr int M = DIM + 2;
+  const double half = 0.5;
+  double velL[DIM], velR[DIM];
+  const double rL = qL[0], rR = qR[0];
+  double v2L = 0., v2R = 0.;
+  for (int d = 0; d < DIM; ++d) {
+    velL[d] = qL[1 + d] / (rL + kEs);
+    velR[d] = qR[1 + d] / (rR + kEs);
+    v2L += velL[d] * velL[d];
+    v2R += velR[d] * velR[d];
+  }
+  const double unL = velL[A], unR = velR[A];
+  const double pL = (g - 1.) * (qL[M - 1] - half * rL * v2L);
+  const double pR = (g - 1.) * (qR[M - 1] - half * rR * v2R);
+  const double HL = (qL[M - 1] + pL) / rL, HR = (qR[M - 1] + pR) / rR;
+  double FL[M], FR[M];
+  FL[0] = rL * unL;
+  FR[0] = rR * unR;
+  for (int d = 0; d < DIM; ++d) {
+    FL[1 + d] = rL * unL * velL[d] + (d == A ? pL : 0.);
+    FR[1 + d] = rR * unR * velR[d] + (d == A ? pR : 0.);
+  }
+  FL[M - 1] = rL * unL * HL;
+  FR[M - 1] = rR * unR * HR;
+  const double RT = sqrt(rR / rL);
+  const double inv = 1. / (1. + RT);
+  double vr2 = 0.;
+  for (int d = 0; d < DIM; ++d) {
+    const double v = (velL[d] + RT * velR[d]) * inv;
+    vr2 += v * v;
+  }
+  const double H = (HL + RT * HR) * inv;
+  const double a = sqrt((g - 1.) * (H - half * vr2));
+  const double smax = (DIM == 1 ? fabs((velL[0] + RT * velR[0]) * inv) : sqrt(vr2)) + a;
+  for (int i = 0; i < M; ++i) F[i] = half * (FL[i] + FR[i] + smax * (qL[i] - qR[i]));
+}
+
+// exact Jacobians of the Rusanov flux w.r.t. the left and right states,
+// including the derivative of the Roe-averaged wave speed,
+// impl/euler_rusanov_flux_jacobian_function.hpp:54-406.  JL, JR row-major MxM.
+template <int DIM, int A>
+__device__ __forceinline__ void euler_fjac(const double *qL, const double *qR, double gamma,
+                                           double *JL, double *JR) {
+  constexpr int M = DIM + 2;
+  const double half = 0.5, gm1 = gamma - 1.;
+  const double *Q[2] = {qL, qR};
+  double vel[2][DIM], V2[2], H[2], aa[2], un[2];
+  for (int s = 0; s < 2; ++s) {
+    const double r = Q[s][0];
+    V2[s] = 0.;
+    for (int d = 0; d < DIM; ++d) {
+      vel[s][d] = Q[s][1 + d] / (r + kEs);
+      V2[s] += vel[s][d] * vel[s][d];
+    }
+    un[s] = vel[s][A];
+    const double p = gm1 * (Q[s][M - 1] - half * r * V2[s]);
+    H[s] = (Q[s][M - 1] + p) / r;
+    aa[s] = gm1 * (H[s] - half * V2[s]); // a^2 of the side
+  }
+  const double rL = qL[0], rR = qR[0];
+  const double r = sqrt(rR * rL), RT = sqrt(rR / rL);
+  const double inv = 1. / (1. + RT);
+  double vR[DIM];
+  double vr2 = 0.;
+  for (int d = 0; d < DIM; ++d) {
+    vR[d] = (vel[0][d] + RT * vel[1][d]) * inv;
+    vr2 += vR[d] * vR[d];
+  }
+  const double Hr = (H[0] + RT * H[1]) * inv;
+  const double a = sqrt(gm1 * (Hr - half * vr2));
+  const double smax = (DIM == 1 ? fabs(vR[0]) : sqrt(vr2)) + a;
+  const double VMagSqrRoe = vr2 + kEs;
+  const double rs = 1. / sqrt(VMagSqrRoe);
+  double rel[DIM];
+  for (int d = 0; d < DIM; ++d) rel[d] = vR[d] * rs;
+  double *JJ[2] = {JL, JR};
+  for (int s = 0; s < 2; ++s) {
+    const double rX = Q[s][0];
+    double VV = 0.;
+    for (int d = 0; d < DIM; ++d) VV += vel[s][d] * vR[d];
+    const double f = 1. / (rX + r);
+    double grad[M];
+    double g0 = 0.;
+    for (int d = 0; d < DIM; ++d) g0 -= half * (vel[s][d] + vR[d]) * rel[d];
+    g0 += half * gm1 / a *
+          (half * (VMagSqrRoe + VV) + half * (H[s] - Hr) - aa[s] / gm1 + half * (gamma - 2.) * V2[s]);
+    grad[0] = f * g0;
+    for (int d = 0; d < DIM; ++d) grad[1 + d] = f * (rel[d] - half * (gm1 * (vR[d] + gm1 * vel[s][d])) / a);
+    grad[M - 1] = half * f * gamma * gm1 / a;
+    double *J = JJ[s];
+    const double uX = un[s];
+    // physical flux Jacobian (times 1/2) for normal e_A
+    J[0] = 0.;
+    for (int d = 0; d < DIM; ++d) J[1 + d] = (d == A ? half : 0.);
+    J[M - 1] = 0.;
+    for (int i = 0; i < DIM; ++i) {
+      double *Ji = J + (1 + i) * M;
+      Ji[0] = half * ((i == A ? half * gm1 * V2[s] : 0.) - vel[s][i] * uX);
+      for (int d = 0; d < DIM; ++d) {
+        double v = (d == A ? vel[s][i] : 0.) - gm1 * vel[s][d] * (i == A ? 1. : 0.);
+        if (d == i) v += uX;
+        Ji[1 + d] = half * v;
+      }
+      Ji[M - 1] = (i == A ? half * gm1 : 0.);
+    }
+    double *Je = J + (M - 1) * M;
+    Je[0] = half * ((half * gm1 * V2[s] - H[s]) * uX);
+    for (int d = 0; d < DIM; ++d) Je[1 + d] = half * ((d == A ? H[s] : 0.) - gm1 * vel[s][d] * uX);
+    Je[M - 1] = half * gamma * uX;
+    for (int i = 0; i < M; ++i) {
+      J[i * M + i] += (s == 0 ? half * smax : -half * smax);
+      const double dq = qL[i] - qR[i];
+      for (int j = 0; j < M; ++j) J[i * M + j] += half * grad[j] * dq;
+    }
+  }
+}
+
+// ------------------------------------------------------- shallow water
+// impl/swe_rusanov_flux_values_function.hpp:54-97
+template <int A>
+__device__ __forceinline__ void swe_flux(const double *qL, const double *qR, double g, double *F) {
+  const double hL = qL[0], uL = qL[1] / (hL + kEs), vL = qL[2] / (hL + kEs);
+  const double hR = qR[0], uR = qR[1] / (hR + kEs), vR = qR[2] / (hR + kEs);
+  const double unL = A == 0 ? uL : vL, unR = A == 0 ? uR : vR;
+  const double pL = 0.5 * g * hL * hL, pR = 0.5 * g * hR * hR;
+  double FL[3], FR[3];
+  FL[0] = hL * unL; FL[1] = hL * unL * uL + (A == 0 ? pL : 0.); FL[2] = hL * unL * vL + (A == 1 ? pL : 0.);
+  FR[0] = hR * unR; FR[1] = hR * unR * uR + (A == 0 ? pR : 0.); FR[2] = hR * unR * vR + (A == 1 ? pR : 0.);
+  const double hm = 0.5 * (hL + hR);
+  const double sL = sqrt(hL), sR = sqrt(hR);
+  const double um = (unL * sL + unR * sR) / (sL + sR + kEs);
+  const double c = sqrt(g * hm);
+  const double smax = sqrt(um * um) + sqrt(c * c);
+  for (int i = 0; i < 3; ++i) F[i] = 0.5 * (FL[i] + FR[i] + smax * (qL[i] - qR[i]));
+}
+
+// impl/swe_rusanov_flux_jacobian_function.hpp:54-137
+template <int A>
+__device__ __forceinline__ void swe_fjac(const double *qL, const double *qR, double g,
+                                         double *JL, double *JR) {
+  const double n0 = A == 0 ? 1. : 0., n1 = A == 1 ? 1. : 0.;
+  const double hL = qL[0], uL = qL[1] / (hL + kEs), vL = qL[2] / (hL + kEs);
+  const double hR = qR[0], uR = qR[1] / (hR + kEs), vR = qR[2] / (hR + kEs);
+  const double unL = A == 0 ? uL : vL, unR = A == 0 ? uR : vR;
+  const double hm = 0.5 * (hL + hR);
+  const double sL = sqrt(hL), sR = sqrt(hR);
+  const double um = (unL * sL + unR * sR) / (sL + sR + kEs);
+  const double smax = fabs(um) + fabs(sqrt(g * hm));
+  const double termL = (n0 * qL[1] + n1 * qL[2]) / (qL[0] * qL[0]);
+  const double termR = (n0 * qR[1] + n1 * qR[2]) / (qR[0] * qR[0]);
+  const double hsun = sL * unL + sR * unR + kEs;
+  const double ahs = fabs(hsun), ssum = sL + sR;
+  const double gterm = g / (2.8284271247461903 * sqrt(g * (hL + hR)));
+  double dL[3], dR[3];
+  dL[0] = -ahs / (2. * sL * (ssum * ssum)) + (0.5 * unL / sL - sL * termL) * hsun / (ssum * ahs) + gterm;
+  dL[1] = n0 * hsun / (sL * ssum * ahs);
+  dL[2] = n1 * hsun / (sL * ssum * ahs);
+  dR[0] = -ahs / (2. * sR * (ssum * ssum)) + (0.5 * unR / sR - sR * termR) * hsun / (ssum * ahs) + gterm;
+  dR[1] = n0 * hsun / (sR * ssum * ahs);
+  dR[2] = n1 * hsun / (sR * ssum * ahs);
+  const double dq0 = qR[0] - qL[0], dq1 = qR[1] - qL[1], dq2 = qR[2] - qL[2];
+  JL[0] = -0.5 * dL[0] * dq0 + 0.5 * (n0 * uL + n1 * vL - qL[0] * termL) + 0.5 * smax;
+  JL[1] = 0.5 * n0 - 0.5 * dL[1] * dq0;
+  JL[2] = 0.5 * n1 - 0.5 * dL[2] * dq0;
+  JL[3] = 0.5 * (g * n0 * qL[0] - qL[1] * termL) - 0.5 * dL[0] * dq1;
+  JL[4] = n0 * uL + 0.5 * n1 * vL + 0.5 * smax - 0.5 * dL[1] * dq1;
+  JL[5] = 0.5 * n1 * uL - 0.5 * dL[2] * dq1;
+  JL[6] = 0.5 * (g * n1 * qL[0] - qL[2] * termL) - 0.5 * dL[0] * dq2;
+  JL[7] = 0.5 * n0 * vL - 0.5 * dL[1] * dq2;
+  JL[8] = n1 * vL + 0.5 * n0 * uL + 0.5 * smax - 0.5 * dL[2] * dq2;
+  JR[0] = -0.5 * dR[0] * dq0 + 0.5 * (n0 * uR + n1 * vR - qR[0] * termR) - 0.5 * smax;
+  JR[1] = 0.5 * n0 - 0.5 * dR[1] * dq0;
+  JR[2] = 0.5 * n1 - 0.5 * dR[2] * dq0;
+  JR[3] = 0.5 * (g * n0 * qR[0] - qR[1] * termR) - 0.5 * dR[0] * dq1;
+  JR[4] = n0 * uR + 0.5 * n1 * vR - 0.5 * smax - 0.5 * dR[1] * dq1;
+  JR[5] = 0.5 * n1 * uR - 0.5 * dR[2] * dq1;
+  JR[6] = 0.5 * (g * n1 * qR[0] - qR[2] * termR) - 0.5 * dR[0] * dq2;
+  JR[7] = 0.5 * n0 * vR - 0.5 * dR[1] * dq2;
+  JR[8] = n1 * vR + 0.5 * n0 * uR - 0.5 * smax - 0.5 * dR[2] * dq2;
+}
+
+// ------------------------------------------------------------ Burgers
+// impl/advection_diffusion_2d_flux_functions.hpp:54-114
+template <int A>
+__device__ __forceinline__ void burgers_flux(const double *qL, const double *qR, double *F) {
+  const double a0 = fmax(fabs(qL[0]), fabs(qR[0]));
+  const double a1 = fmax(fabs(qL[1]), fabs(qR[1]));
+  const double cross = qL[0] * qL[1] + qR[0] * qR[1];
+  double f0 = a0 * (qL[0] - qR[0]);
+  double f1 = a1 * (qL[1] - qR[1]);
+  if (A == 0) {
+    f0 += qL[0] * qL[0] + qR[0] * qR[0];
+    f1 += cross;
+  } else {
+    f0 += cross;
+    f1 += qL[1] * qL[1] + qR[1] * qR[1];
+  }
+  F[0] = f0 * 0.25;
+  F[1] = f1 * 0.25;
+}
+
+template <int A>
+__device__ __forceinline__ void burgers_fjac(const double *qL, const double *qR, double *JL,
+                                             double *JR) {
+  const double n0 = A == 0 ? 1. : 0., n1 = A == 1 ? 1. : 0.;
+  if (fabs(qL[0]) > fabs(qR[0])) {
+    JL[0] = (2. * qL[0] - qR[0]) * copysign(1., qL[0]) + n0 * 2. * qL[0] + n1 * qL[1];
+    JR[0] = n0 * 2. * qR[0] + n1 * qR[1] - fabs(qL[0]);
+  } else {
+    JL[0] = n0 * 2. * qL[0] + n1 * qL[1] + fabs(qR[0]);
+    JR[0] = (qL[0] - 2. * qR[0]) * copysign(1., qR[0]) + n0 * 2. * qR[0] + n1 * qR[1];
+  }
+  if (fabs(qL[1]) > fabs(qR[1])) {
+    JL[3] = (2. * qL[1] - qR[1]) * copysign(1., qL[1]) + n0 * qL[0] + n1 * 2. * qL[1];
+    JR[3] = n0 * qR[0] + n1 * 2. * qR[1] - fabs(qL[1]);
+  } else {
+    JL[3] = n0 * qL[0] + n1 * 2. * qL[1] + fabs(qR[1]);
+    JR[3] = (qL[1] - 2. * qR[1]) * copysign(1., qR[1]) + n0 * qR[0] + n1 * 2. * qR[1];
+  }
+  JL[0] *= 0.25; JR[0] *= 0.25; JL[3] *= 0.25; JR[3] *= 0.25;
+  JL[1] = n1 * qL[0] * 0.25;
+  JL[2] = n0 * qL[1] * 0.25;
+  JR[1] = n1 * qR[0] * 0.25;
+  JR[2] = n0 * qR[1] * 0.25;
+}
+
+}  // namespace phys
+}  // namespace pda

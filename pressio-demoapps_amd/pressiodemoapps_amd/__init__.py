@@ -1,0 +1,428 @@
+"""MI355X-native pressio-demoapps finite-volume engine: Python host mirror.
+
+This module mirrors the reference Python API (pressiodemoapps/__init__.py and
+src_py/main_binder.cc): ``load_cellcentered_uniform_mesh``, ``create_problem``
+and the problem methods ``rightHandSide`` / ``jacobian`` /
+``rightHandSideAndJacobian`` / ``applyJacobian`` / ``initialCondition``.
+Everything evaluates through the C ABI of ``libpda.so`` (include/pda_c_api.h)
+on HIP device memory; state, velocity and Jacobian values are torch tensors on
+``cuda:0`` used as plain device buffers.  There is no CPU fallback: a missing
+library or GPU raises.
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBPATH = os.path.join(os.path.dirname(_HERE), "libpda.so")
+_lib = None
+
+
+class PdaError(RuntimeError):
+    pass
+
+
+def library_path():
+    return _LIBPATH
+
+
+def lib():
+    """Load libpda.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIBPATH):
+            raise PdaError(f"libpda.so not built at {_LIBPATH}; run __graft_entry__.build()")
+        L = ctypes.CDLL(_LIBPATH)
+        i32, i64, dbl, vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+        P = ctypes.POINTER
+        sig = {
+            "pda_last_error": ([], ctypes.c_char_p),
+            "pda_version": ([], ctypes.c_char_p),
+            "pda_mesh_load": ([ctypes.c_char_p, P(vp)], i32),
+            "pda_mesh_create_structured": ([i32, P(i64), P(dbl), i32, i32, P(vp)], i32),
+            "pda_mesh_create_sample": ([vp, P(ctypes.c_int32), i64, P(vp)], i32),
+            "pda_mesh_sample_gids": ([vp, P(ctypes.c_int32), P(ctypes.c_int32)], i32),
+            "pda_mesh_write": ([vp, ctypes.c_char_p], i32),
+            "pda_mesh_info": ([vp, P(i32), P(i32), P(i64), P(i64), P(dbl)], i32),
+            "pda_mesh_graph": ([vp], P(ctypes.c_int32)),
+            "pda_mesh_coords": ([vp, i32], P(dbl)),
+            "pda_mesh_row_classes": ([vp, P(i64), P(i64)], i32),
+            "pda_mesh_destroy": ([vp], None),
+            "pda_problem_create": ([vp, i32, i32, i32, i32, P(ctypes.c_char_p), P(dbl), i32, P(vp)], i32),
+            "pda_problem_destroy": ([vp], None),
+            "pda_problem_sizes": ([vp, P(i32), P(i64), P(i64), P(i64)], i32),
+            "pda_problem_parameter": ([vp, ctypes.c_char_p, P(dbl)], i32),
+            "pda_initial_condition": ([vp, vp, vp], i32),
+            "pda_jacobian_pattern": ([vp, P(i64), P(i64), P(vp), P(vp)], i32),
+            "pda_jacobian_pattern_host": ([vp, P(i64), P(ctypes.c_int32)], i32),
+            "pda_velocity": ([vp, vp, dbl, vp, vp], i32),
+            "pda_jacobian": ([vp, vp, dbl, vp, vp, vp], i32),
+            "pda_apply_jacobian": ([vp, vp, dbl, vp, i64, i32, vp, vp], i32),
+            "pda_jacobian_blocks": ([vp, P(i32), P(vp), P(vp)], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().pda_last_error().decode()
+        raise PdaError(f"pda error {rc}: {msg}")
+
+
+# ------------------------------------------------------------------ enums
+class InviscidFluxReconstruction(enum.IntEnum):
+    FirstOrder = 0
+    Weno3 = 1
+    Weno5 = 2
+
+
+class Euler1d(enum.IntEnum):
+    PeriodicSmooth = 0
+    Sod = 1
+    Lax = 2
+    ShuOsher = 3
+
+
+class Euler2d(enum.IntEnum):
+    PeriodicSmooth = 0
+    KelvinHelmholtz = 1
+    SedovFull = 2
+    SedovSymmetry = 3
+    Riemann = 4
+    NormalShock = 5
+    DoubleMachReflection = 6
+    CrossShock = 7
+    testingonlyneumann = 8
+
+
+class Euler3d(enum.IntEnum):
+    PeriodicSmooth = 0
+    SedovSymmetry = 1
+
+
+class Swe2d(enum.IntEnum):
+    SlipWall = 0
+    CustomBCs = 1
+
+
+class AdvectionDiffusion2d(enum.IntEnum):
+    BurgersPeriodic = 0
+    BurgersOutflow = 1
+
+
+class AdvectionDiffusionReaction2d(enum.IntEnum):
+    ProblemA = 0
+
+
+FAMILY = {Euler1d: 1, Euler2d: 2, Euler3d: 3, Swe2d: 4,
+          AdvectionDiffusion2d: 5, AdvectionDiffusionReaction2d: 6}
+
+
+# ------------------------------------------------------------------- mesh
+class CellCenteredUniformMesh:
+    """Cell-centred uniform mesh (impl/mesh_ccu.hpp:60-473)."""
+
+    def __init__(self, handle):
+        self._h = ctypes.c_void_p(handle)
+        L = lib()
+        dim, st = ctypes.c_int(), ctypes.c_int()
+        ns, nst = ctypes.c_int64(), ctypes.c_int64()
+        d = (ctypes.c_double * 3)()
+        _check(L.pda_mesh_info(self._h, ctypes.byref(dim), ctypes.byref(st), ctypes.byref(ns),
+                               ctypes.byref(nst), d))
+        self._dim, self._stencil = dim.value, st.value
+        self._ns, self._nst = ns.value, nst.value
+        self._d = tuple(d)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.pda_mesh_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def dimensionality(self):
+        return self._dim
+
+    def stencilSize(self):
+        return self._stencil
+
+    def sampleMeshSize(self):
+        return self._ns
+
+    def stencilMeshSize(self):
+        return self._nst
+
+    def dx(self):
+        return self._d[0]
+
+    def dy(self):
+        return self._d[1]
+
+    def dz(self):
+        return self._d[2]
+
+    def graph(self):
+        ptr = lib().pda_mesh_graph(self._h)
+        nc = 1 + (self._stencil - 1) * self._dim
+        return np.ctypeslib.as_array(ptr, shape=(self._ns, nc)).copy()
+
+    def _coords(self, axis):
+        ptr = lib().pda_mesh_coords(self._h, axis)
+        return np.ctypeslib.as_array(ptr, shape=(self._nst,)).copy()
+
+    def viewX(self):
+        return self._coords(0)
+
+    def viewY(self):
+        return self._coords(1)
+
+    def viewZ(self):
+        return self._coords(2)
+
+    def row_classes(self):
+        ni, nn = ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().pda_mesh_row_classes(self._h, ctypes.byref(ni), ctypes.byref(nn)))
+        return ni.value, nn.value
+
+    def write(self, directory):
+        os.makedirs(directory, exist_ok=True)
+        _check(lib().pda_mesh_write(self._h, directory.encode()))
+
+    def sample_gids(self):
+        """(sample cells, stencil cells) as full-mesh ids, for sample meshes."""
+        s = np.zeros(self._ns, dtype=np.int32)
+        t = np.zeros(self._nst, dtype=np.int32)
+        _check(lib().pda_mesh_sample_gids(
+            self._h, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+            t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return s, t
+
+
+def load_cellcentered_uniform_mesh(path):
+    h = ctypes.c_void_p()
+    _check(lib().pda_mesh_load(str(path).encode(), ctypes.byref(h)))
+    return CellCenteredUniformMesh(h.value)
+
+
+def create_structured_mesh(n, bounds, stencil, periodic=()):
+    """Full natural-order mesh as meshing_scripts/create_full_mesh.py writes it.
+
+    n: cells per axis (len = dim); bounds: (xmin, xmax[, ymin, ymax[, zmin, zmax]]);
+    periodic: axes (0,1,2) with periodic wrap, or True for all.
+    """
+    dim = len(n)
+    nn = (ctypes.c_int64 * 3)(*(list(n) + [1] * (3 - dim)))
+    b = (ctypes.c_double * 6)(*(list(bounds) + [0.0] * (6 - len(bounds))))
+    if periodic is True:
+        periodic = tuple(range(dim))
+    mask = 0
+    for a in periodic:
+        mask |= 1 << a
+    h = ctypes.c_void_p()
+    _check(lib().pda_mesh_create_structured(dim, nn, b, stencil, mask, ctypes.byref(h)))
+    return CellCenteredUniformMesh(h.value)
+
+
+def create_sample_mesh(full_mesh, sample_gids):
+    g = np.ascontiguousarray(np.asarray(sample_gids, dtype=np.int32))
+    h = ctypes.c_void_p()
+    _check(lib().pda_mesh_create_sample(full_mesh.handle,
+                                        g.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        len(g), ctypes.byref(h)))
+    return CellCenteredUniformMesh(h.value)
+
+
+# ---------------------------------------------------------------- problem
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(_torch().cuda.current_stream().cuda_stream)
+
+
+def _dev_tensor(t, n, name):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda or t.dtype != torch.float64:
+        raise PdaError(f"{name}: expected a float64 CUDA (HIP) tensor")
+    if not t.is_contiguous() or t.numel() != n:
+        raise PdaError(f"{name}: expected a contiguous tensor of {n} entries, got {t.numel()}")
+    return t
+
+
+class JacobianCSR:
+    """Row-major CSR Jacobian with a fixed pattern (device tensors)."""
+
+    def __init__(self, rowptr, colidx, values, shape):
+        self.rowptr, self.colidx, self.values, self.shape = rowptr, colidx, values, shape
+
+    def nonZeros(self):
+        return self.values.numel()
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.values.cpu().numpy(), self.colidx.cpu().numpy(),
+                              self.rowptr.cpu().numpy()), shape=self.shape)
+
+
+class Problem:
+    """Mirror of PublicProblemEigenMixinCpp (adapter_cpp.hpp:60-264) on device."""
+
+    def __init__(self, mesh, problem_enum, scheme=InviscidFluxReconstruction.FirstOrder,
+                 ic_flag=0, params=None):
+        fam = FAMILY[type(problem_enum)]
+        self._mesh = mesh
+        keys = list((params or {}).keys())
+        vals = [float(params[k]) for k in keys]
+        ck = (ctypes.c_char_p * max(len(keys), 1))(*[k.encode() for k in keys])
+        cv = (ctypes.c_double * max(len(vals), 1))(*vals)
+        h = ctypes.c_void_p()
+        _check(lib().pda_problem_create(mesh.handle, fam, int(problem_enum), int(scheme),
+                                        int(ic_flag), ck, cv, len(keys), ctypes.byref(h)))
+        self._h = h
+        m, ns, nst, nnz = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _check(lib().pda_problem_sizes(h, ctypes.byref(m), ctypes.byref(ns), ctypes.byref(nst),
+                                       ctypes.byref(nnz)))
+        self._m, self._ns, self._nst, self._nnz = m.value, ns.value, nst.value, nnz.value
+        self._pattern = None
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.pda_problem_destroy(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def numDofPerCell(self):
+        return self._m
+
+    def totalDofSampleMesh(self):
+        return self._ns
+
+    def totalDofStencilMesh(self):
+        return self._nst
+
+    def nonZeros(self):
+        return self._nnz
+
+    def queryParameter(self, name):
+        v = ctypes.c_double()
+        _check(lib().pda_problem_parameter(self._h, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def gamma(self):
+        return self.queryParameter("gamma")
+
+    def gravity(self):
+        return self.queryParameter("gravity")
+
+    def coriolis(self):
+        return self.queryParameter("coriolis")
+
+    def initialCondition(self):
+        torch = _torch()
+        U = torch.empty(self._nst, dtype=torch.float64, device="cuda")
+        _check(lib().pda_initial_condition(self._h, _ptr(U), _stream()))
+        return U
+
+    def createState(self):
+        return _torch().zeros(self._nst, dtype=_torch().float64, device="cuda")
+
+    def createRightHandSide(self):
+        return _torch().zeros(self._ns, dtype=_torch().float64, device="cuda")
+
+    createVelocity = createRightHandSide
+
+    def jacobian_pattern_host(self):
+        """(rowptr int64, colidx int32) as numpy, reference CSR order."""
+        if self._pattern is None:
+            rp = np.zeros(self._ns + 1, dtype=np.int64)
+            ci = np.zeros(self._nnz, dtype=np.int32)
+            _check(lib().pda_jacobian_pattern_host(
+                self._h, rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                ci.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+            self._pattern = (rp, ci)
+        return self._pattern
+
+    def createJacobian(self):
+        torch = _torch()
+        rp, ci = self.jacobian_pattern_host()
+        return JacobianCSR(torch.from_numpy(rp).cuda(), torch.from_numpy(ci).cuda(),
+                           torch.zeros(self._nnz, dtype=torch.float64, device="cuda"),
+                           (self._ns, self._nst))
+
+    def rightHandSide(self, U, t, V):
+        _dev_tensor(U, self._nst, "state")
+        _dev_tensor(V, self._ns, "rhs")
+        _check(lib().pda_velocity(self._h, _ptr(U), float(t), _ptr(V), _stream()))
+
+    velocity = rightHandSide
+    rhs = rightHandSide
+    __call__ = rightHandSide
+
+    def _jvals(self, J):
+        vals = J.values if isinstance(J, JacobianCSR) else J
+        return _dev_tensor(vals, self._nnz, "jacobian values")
+
+    def jacobian(self, U, t, J):
+        _dev_tensor(U, self._nst, "state")
+        _check(lib().pda_jacobian(self._h, _ptr(U), float(t), _ptr(self._jvals(J)), None,
+                                  _stream()))
+
+    def rightHandSideAndJacobian(self, U, t, V, J):
+        _dev_tensor(U, self._nst, "state")
+        _dev_tensor(V, self._ns, "rhs")
+        _check(lib().pda_jacobian(self._h, _ptr(U), float(t), _ptr(self._jvals(J)), _ptr(V),
+                                  _stream()))
+
+    def createApplyJacobianResult(self, B):
+        torch = _torch()
+        if B.dim() == 1:
+            return torch.zeros(self._ns, dtype=torch.float64, device="cuda")
+        return torch.zeros(self._ns, B.shape[1], dtype=torch.float64, device="cuda")
+
+    def applyJacobian(self, U, B, t, AJ):
+        """AJ = J(U) B for a vector or a (column- or row-major) matrix B."""
+        _dev_tensor(U, self._nst, "state")
+        if B.dim() == 1:
+            ncols, layout = 1, 0
+            Bc = B.contiguous()
+        else:
+            ncols = B.shape[1]
+            if B.is_contiguous():
+                layout, Bc = 1, B
+            else:
+                layout, Bc = 0, B.t().contiguous()
+        out = AJ if (AJ.is_contiguous() and (B.dim() == 1 or layout == 1)) else None
+        tgt = out if out is not None else _torch().empty(
+            (ncols, self._ns) if layout == 0 else (self._ns, ncols), dtype=_torch().float64,
+            device="cuda")
+        _check(lib().pda_apply_jacobian(self._h, _ptr(U), float(t), _ptr(Bc), ncols, layout,
+                                        _ptr(tgt), _stream()))
+        if out is None:
+            AJ.copy_(tgt.t() if layout == 0 else tgt)
+
+
+def create_problem(mesh, problem_enum, scheme=InviscidFluxReconstruction.FirstOrder,
+                   ic_flag=0, params=None):
+    """create_problem(mesh, ProblemEnum, InviscidFluxReconstruction[, icFlag][, params])."""
+    if isinstance(ic_flag, dict):
+        params, ic_flag = ic_flag, 0
+    return Problem(mesh, problem_enum, scheme, ic_flag, params)
