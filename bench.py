@@ -1,0 +1,194 @@
+"""Benchmark of the finite-volume hot path on one MI355X.
+
+Workload (the headline configuration): 2D Euler four-quadrant Riemann problem
+(Riemann IC2), 1024 x 1024 cells on [0,1]^2, gamma 1.4, stencil-7 mesh, FP64.
+One step = one seeded +-1% perturbation U*(1+delta) of the initial condition,
+evaluated as WENO5 velocity() (problem A) plus first-order exact-Rusanov
+jacobian() (problem B, 4x4 blocks, 5-point stencil), all inputs resident in
+HBM before the timed region.  Metric: residual+Jacobian cells/s.
+
+Prints one JSON line (the last line of stdout).
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "pressio-demoapps_amd"), os.path.join(ROOT, "oracle")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n_cpu, budget_s):
+    """Reference CPU path (oracle/_ref, OpenMP) or the C restatement, timed on
+    this host on an n_cpu x n_cpu sample of the same workload."""
+    import numpy as np
+    import pyoracle
+    import pressiodemoapps_amd as pda
+    mesh = pda.create_structured_mesh((n_cpu, n_cpu), (0.0, 1.0, 0.0, 1.0), 7)
+    cells = n_cpu * n_cpu
+    rng = np.random.default_rng(0)
+    threads = os.cpu_count() or 1
+    if pyoracle.ref_available(omp=True):
+        d = tempfile.mkdtemp(prefix="pda_bench_mesh_")
+        mesh.write(d)
+        ra = pyoracle.RefProblem(d, 2, 4, 2, 2, omp=True)
+        rb = pyoracle.RefProblem(d, 2, 4, 0, 2, omp=True)
+        U0 = ra.initial_condition()
+        kind = "reference"
+        velocity = lambda U: ra.velocity(U)  # noqa: E731
+        jacobian = lambda U: rb.velocity_and_jacobian(U)  # noqa: E731
+        threads = int(os.environ.get("OMP_NUM_THREADS", threads))
+    else:
+        orc = pyoracle.Oracle()
+        ms = orc.from_mesh(mesh)
+        import ctypes
+        pa = pda.create_problem(mesh, pda.Euler2d.Riemann, pda.InviscidFluxReconstruction.Weno5, 2)
+        U0 = np.zeros(pa.totalDofStencilMesh())
+        pda.lib().pda_initial_condition_host(pa.handle, U0.ctypes.data_as(ctypes.c_void_p))
+        pba, pbb = orc.problem(2, 4, 2, [1.4]), orc.problem(2, 4, 0, [1.4])
+        kind = "port"
+        threads = 1
+        velocity = lambda U: orc.velocity(ms, pba, U)  # noqa: E731
+        jacobian = lambda U: orc.velocity_and_jacobian(ms, pbb, U)  # noqa: E731
+    U = U0 * (1.0 + 0.01 * rng.uniform(-1, 1, U0.shape))
+    velocity(U)
+    jacobian(U)  # warm-up
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        velocity(U)
+        jacobian(U)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or reps >= 200:
+            break
+    return {"value": reps * cells / el, "unit": "cells/s", "cores": threads, "kind": kind,
+            "sample": f"{n_cpu}x{n_cpu} Riemann IC2 (same mesh/scheme), {reps} perturbations of "
+                      f"WENO5 rightHandSide + first-order jacobian in {el:.1f} s, {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--cpu-n", type=int, default=512)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
+                                   "(the path shards by mesh slabs, see DESIGN.md)"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import pressiodemoapps_amd as pda
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    n = args.n
+    log(f"[bench] building {n}x{n} mesh and problems")
+    mesh = pda.create_structured_mesh((n, n), (0.0, 1.0, 0.0, 1.0), 7)
+    probA = pda.create_problem(mesh, pda.Euler2d.Riemann, pda.InviscidFluxReconstruction.Weno5, 2)
+    probB = pda.create_problem(mesh, pda.Euler2d.Riemann, pda.InviscidFluxReconstruction.FirstOrder, 2)
+    cells = n * n
+    m = probA.numDofPerCell()
+    U0 = probA.initialCondition()
+    nst = probA.totalDofStencilMesh()
+    npert = min(args.steps + args.warmup, 100)
+    states = []
+    for i in range(npert):
+        g = torch.Generator(device="cuda").manual_seed(i)
+        d = torch.rand(nst, dtype=torch.float64, device="cuda", generator=g) * 2.0 - 1.0
+        states.append(U0 * (1.0 + 0.01 * d))
+    V = probA.createRightHandSide()
+    J = probB.createJacobian()
+    nnz = J.nonZeros()
+    torch.cuda.synchronize()
+
+    def step(i):
+        U = states[i % npert]
+        probA.rightHandSide(U, 0.0, V)
+        probB.jacobian(U, 0.0, J)
+
+    log(f"[bench] warmup {args.warmup}")
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+
+    log(f"[bench] timing {args.steps} steps")
+    s = torch.cuda.current_stream()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True),
+           torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        U = states[(args.warmup + i) % npert]
+        ev[i][0].record(s)
+        probA.rightHandSide(U, 0.0, V)
+        ev[i][1].record(s)
+        probB.jacobian(U, 0.0, J)
+        ev[i][2].record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_v = sum(a.elapsed_time(b) for a, b, _ in ev) / args.steps * 1e-3
+    t_j = sum(b.elapsed_time(c) for _, b, c in ev) / args.steps * 1e-3
+    ms_per_step = wall / args.steps * 1e3
+
+    # algorithmic bytes: velocity 8*m*(Nst+N); Jacobian 8*nnz + 8*m*Nst
+    bytes_v = 8 * m * (cells + cells)
+    bytes_j = 8 * nnz + 8 * m * cells
+    ach_j = bytes_j / t_j / 1e9
+    value = cells / (wall / args.steps)
+    log(f"[bench] velocity {t_v*1e6:.1f} us ({bytes_v/t_v/1e9:.0f} GB/s), jacobian {t_j*1e6:.1f} us "
+        f"({ach_j:.0f} GB/s), step {ms_per_step*1e3:.1f} us, unit bytes {(bytes_v+bytes_j)/1e6:.0f} MB "
+        f"-> {(bytes_v+bytes_j)/(wall/args.steps)/1e9:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] cpu baseline")
+        try:
+            cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
+        except Exception as e:  # reported, never fatal
+            log(f"[bench] cpu baseline failed: {e}")
+    out = {
+        "metric": "residual+Jacobian cells/s",
+        "value": value,
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: Riemann IC2 with seeded +-1% uniform perturbations",
+        "config": {"workload": f"2D Euler Riemann IC2 {n}x{n}, WENO5 velocity + first-order "
+                               f"jacobian per perturbation", "cells": cells, "jacobian_nnz": nnz,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": ach_j, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": ach_j / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "first-order jacobian (k_jacobian)",
+                     "bytes_per_launch": bytes_j, "avg_launch_us": t_j * 1e6},
+        "unit_bytes_per_step": bytes_v + bytes_j,
+        "hbm_frac_per_step": (bytes_v + bytes_j) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
+        "velocity_us": t_v * 1e6,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -115,6 +115,8 @@ int pda_problem_parameter(pda_problem p, const char *name, double *value);
 
 /* initialCondition(), into device memory of n_stencil_dofs doubles */
 int pda_initial_condition(pda_problem p, double *d_U, pda_stream stream);
+/* the same into host memory (no device needed) */
+int pda_initial_condition_host(pda_problem p, double *U);
 
 /* Jacobian sparsity, fixed at creation: scalar CSR with the reference's row
    order and sorted columns (Eigen setFromTriplets + makeCompressed,

@@ -138,6 +138,12 @@ struct pda_problem_s {
   int64_t nnz = 0;
   std::vector<int64_t> rowptr;       // host copy of the scalar CSR
   std::vector<int32_t> colidx;
+  // host tables (built at creation, uploaded on first evaluation)
+  std::vector<int32_t> h_ghost_row, h_near_rows, h_nblk, h_bcol;
+  std::vector<int64_t> h_rowbase;
+  std::vector<uint8_t> h_slot;
+  std::vector<double> h_bcf;
+  bool on_device = false;
   pda::View view;
   pda::DevBuf d_graph, d_ghost_row, d_near_rows, d_ghost, d_bcf;
   pda::DevBuf d_rowptr, d_colidx, d_rowbase, d_nblk, d_slot, d_bcol;
@@ -145,6 +151,7 @@ struct pda_problem_s {
 };
 
 namespace pda {
+int ensure_device(pda_problem_s &p);
 int launch_ghosts(const pda_problem_s &p, const double *U, hipStream_t s);
 int launch_velocity(const pda_problem_s &p, const double *U, double t, double *V, hipStream_t s);
 int launch_jacobian(const pda_problem_s &p, const double *U, double t, double *J,

@@ -331,35 +331,17 @@ int build(pda_problem_s &p) {
       bc_factors(p, near_rows[g], a, f);
       for (int k = 0; k < m; ++k) bcf[(g * dim + a) * m + k] = f[k];
     }
-  // device copies
-  PDA_HIP_CHECK(p.d_graph.upload(M.graph));
-  PDA_HIP_CHECK(p.d_ghost_row.upload(ghost_row));
-  PDA_HIP_CHECK(p.d_near_rows.upload(near_rows));
-  PDA_HIP_CHECK(p.d_bcf.upload(bcf));
-  PDA_HIP_CHECK(p.d_rowbase.upload(rowbase));
-  PDA_HIP_CHECK(p.d_nblk.upload(nblk));
-  PDA_HIP_CHECK(p.d_slot.upload(slot));
-  PDA_HIP_CHECK(p.d_bcol.upload(bcol));
-  const size_t gbytes = sizeof(double) * size_t(2 * dim) * size_t(std::max<int64_t>(p.n_near, 1)) * w * m;
-  PDA_HIP_CHECK(p.d_ghost.alloc(gbytes));
-  {
-    // unused ghost slots hold numeric_limits<double>::min(),
-    // impl/euler_2d_prob_class.hpp:1266-1270
-    std::vector<double> init(gbytes / sizeof(double), 2.2250738585072014e-308);
-    PDA_HIP_CHECK(hipMemcpy(p.d_ghost.ptr, init.data(), gbytes, hipMemcpyHostToDevice));
-  }
+  p.h_ghost_row = std::move(ghost_row);
+  p.h_near_rows = std::move(near_rows);
+  p.h_bcf = std::move(bcf);
+  p.h_rowbase = std::move(rowbase);
+  p.h_nblk = std::move(nblk);
+  p.h_slot = std::move(slot);
+  p.h_bcol = std::move(bcol);
   View &v = p.view;
-  v.graph = p.d_graph.as<int32_t>();
   v.ncols = nc;
   v.nrows = nr;
-  v.ghost_row = p.d_ghost_row.as<int32_t>();
-  v.near_rows = p.d_near_rows.as<int32_t>();
   v.n_near = p.n_near;
-  v.ghost = p.d_ghost.as<double>();
-  v.bcf = p.d_bcf.as<double>();
-  v.rowbase = p.d_rowbase.as<int64_t>();
-  v.nblk = p.d_nblk.as<int32_t>();
-  v.slot = p.d_slot.as<uint8_t>();
   v.npos = npos;
   v.dup = dup ? 1 : 0;
   for (int a = 0; a < 3; ++a) v.hinv[a] = M.dinv[a];
@@ -378,14 +360,12 @@ int build(pda_problem_s &p) {
   return PDA_OK;
 }
 
-int ensure_scalar_pattern(pda_problem_s &p) {
+int ensure_scalar_pattern_host(pda_problem_s &p) {
   if (!p.rowptr.empty()) return PDA_OK;
   const pda_mesh_s &M = *p.mesh;
   const int m = p.m;
   const int64_t nr = M.n_sample;
-  std::vector<int32_t> nblk(nr), bcol(size_t(nr) * p.maxb);
-  PDA_HIP_CHECK(hipMemcpy(nblk.data(), p.d_nblk.ptr, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
-  PDA_HIP_CHECK(hipMemcpy(bcol.data(), p.d_bcol.ptr, sizeof(int32_t) * bcol.size(), hipMemcpyDeviceToHost));
+  const std::vector<int32_t> &nblk = p.h_nblk, &bcol = p.h_bcol;
   p.rowptr.resize(nr * m + 1);
   p.colidx.resize(p.nnz);
   int64_t e = 0;
@@ -396,12 +376,53 @@ int ensure_scalar_pattern(pda_problem_s &p) {
         for (int j = 0; j < m; ++j) p.colidx[e++] = bcol[size_t(r) * p.maxb + s] * m + j;
       p.rowptr[r * m + k + 1] = e;
     }
-  PDA_HIP_CHECK(p.d_rowptr.upload(p.rowptr));
-  PDA_HIP_CHECK(p.d_colidx.upload(p.colidx));
+  return PDA_OK;
+}
+
+int ensure_scalar_pattern(pda_problem_s &p) {
+  int rc = ensure_scalar_pattern_host(p);
+  if (rc) return rc;
+  rc = ensure_device(p);
+  if (rc) return rc;
+  if (!p.d_rowptr.ptr) {
+    PDA_HIP_CHECK(p.d_rowptr.upload(p.rowptr));
+    PDA_HIP_CHECK(p.d_colidx.upload(p.colidx));
+  }
   return PDA_OK;
 }
 
 }  // namespace
+
+namespace pda {
+// device copies of the index maps, made on the first evaluation
+int ensure_device(pda_problem_s &p) {
+  if (p.on_device) return PDA_OK;
+  const pda_mesh_s &M = *p.mesh;
+  PDA_HIP_CHECK(p.d_graph.upload(M.graph));
+  PDA_HIP_CHECK(p.d_ghost_row.upload(p.h_ghost_row));
+  PDA_HIP_CHECK(p.d_near_rows.upload(p.h_near_rows));
+  PDA_HIP_CHECK(p.d_bcf.upload(p.h_bcf));
+  PDA_HIP_CHECK(p.d_rowbase.upload(p.h_rowbase));
+  PDA_HIP_CHECK(p.d_nblk.upload(p.h_nblk));
+  PDA_HIP_CHECK(p.d_slot.upload(p.h_slot));
+  PDA_HIP_CHECK(p.d_bcol.upload(p.h_bcol));
+  const size_t gn = size_t(2 * p.dim) * size_t(std::max<int64_t>(p.n_near, 1)) * p.w * p.m;
+  // unused ghost slots hold numeric_limits<double>::min(),
+  // impl/euler_2d_prob_class.hpp:1266-1270
+  PDA_HIP_CHECK(p.d_ghost.upload(std::vector<double>(gn, 2.2250738585072014e-308)));
+  View &v = p.view;
+  v.graph = p.d_graph.as<int32_t>();
+  v.ghost_row = p.d_ghost_row.as<int32_t>();
+  v.near_rows = p.d_near_rows.as<int32_t>();
+  v.ghost = p.d_ghost.as<double>();
+  v.bcf = p.d_bcf.as<double>();
+  v.rowbase = p.d_rowbase.as<int64_t>();
+  v.nblk = p.d_nblk.as<int32_t>();
+  v.slot = p.d_slot.as<uint8_t>();
+  p.on_device = true;
+  return PDA_OK;
+}
+}  // namespace pda
 
 extern "C" {
 
@@ -532,9 +553,18 @@ int pda_jacobian_pattern(pda_problem p, int64_t *n_rows, int64_t *nnz, const int
   return PDA_OK;
 }
 
+int pda_initial_condition_host(pda_problem p, double *U) {
+  if (!p || !U) return PDA_E_INVALID_ARG;
+  std::vector<double> v;
+  const int rc = initial_condition_host(*p, v);
+  if (rc) return rc;
+  std::memcpy(U, v.data(), sizeof(double) * v.size());
+  return PDA_OK;
+}
+
 int pda_jacobian_pattern_host(pda_problem p, int64_t *rowptr, int32_t *colidx) {
   if (!p) return PDA_E_INVALID_ARG;
-  const int rc = ensure_scalar_pattern(*p);
+  const int rc = ensure_scalar_pattern_host(*p);
   if (rc) return rc;
   if (rowptr) std::memcpy(rowptr, p->rowptr.data(), sizeof(int64_t) * p->rowptr.size());
   if (colidx) std::memcpy(colidx, p->colidx.data(), sizeof(int32_t) * p->colidx.size());
@@ -546,6 +576,8 @@ int pda_velocity(pda_problem p, const double *d_U, double t, double *d_V, pda_st
     set_error("pda_velocity: invalid argument");
     return PDA_E_INVALID_ARG;
   }
+  const int rc = ensure_device(*p);
+  if (rc) return rc;
   return launch_velocity(*p, d_U, t, d_V, hipStream_t(stream));
 }
 
@@ -555,6 +587,8 @@ int pda_jacobian(pda_problem p, const double *d_U, double t, double *d_J, double
     set_error("pda_jacobian: invalid argument");
     return PDA_E_INVALID_ARG;
   }
+  const int rc = ensure_device(*p);
+  if (rc) return rc;
   return launch_jacobian(*p, d_U, t, d_J, d_V, hipStream_t(stream));
 }
 
@@ -576,6 +610,8 @@ int pda_apply_jacobian(pda_problem p, const double *d_U, double t, const double 
 
 int pda_jacobian_blocks(pda_problem p, int *maxb, const int32_t **d_nblk, const int32_t **d_bcol) {
   if (!p) return PDA_E_INVALID_ARG;
+  const int rc = ensure_device(*p);
+  if (rc) return rc;
   if (maxb) *maxb = p->maxb;
   if (d_nblk) *d_nblk = p->d_nblk.as<int32_t>();
   if (d_bcol) *d_bcol = p->d_bcol.as<int32_t>();

@@ -55,6 +55,7 @@ def lib():
             "pda_problem_sizes": ([vp, P(i32), P(i64), P(i64), P(i64)], i32),
             "pda_problem_parameter": ([vp, ctypes.c_char_p, P(dbl)], i32),
             "pda_initial_condition": ([vp, vp, vp], i32),
+            "pda_initial_condition_host": ([vp, vp], i32),
             "pda_jacobian_pattern": ([vp, P(i64), P(i64), P(vp), P(vp)], i32),
             "pda_jacobian_pattern_host": ([vp, P(i64), P(ctypes.c_int32)], i32),
             "pda_velocity": ([vp, vp, dbl, vp, vp], i32),

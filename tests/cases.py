@@ -1,0 +1,65 @@
+"""Shared problem cases for the parity tests (family, problem enum, scheme,
+mesh shape).  Small enough that the CPU oracle finishes in milliseconds."""
+import numpy as np
+
+import pressiodemoapps_amd as pda
+
+S = pda.InviscidFluxReconstruction
+
+# (id, problem enum, scheme, n, bounds, mesh stencil, periodic, ic flag, params)
+CASES = []
+for s in (S.FirstOrder, S.Weno3, S.Weno5):
+    CASES += [
+        (f"euler1d-sod-{s.name}", pda.Euler1d.Sod, s, (64,), (-0.5, 0.5), 7, (), 0, None),
+        (f"euler1d-periodic-{s.name}", pda.Euler1d.PeriodicSmooth, s, (40,), (-1, 1), 7, True, 0, None),
+        (f"euler2d-riemann-{s.name}", pda.Euler2d.Riemann, s, (24, 20), (0, 1, 0, 1), 7, (), 2, None),
+        (f"euler2d-periodic-{s.name}", pda.Euler2d.PeriodicSmooth, s, (16, 18), (-1, 1, -1, 1), 7, True, 0, None),
+        (f"swe-slipwall-{s.name}", pda.Swe2d.SlipWall, s, (20, 22), (-5, 5, -5, 5), 7, (), 1,
+         {"gravity": 9.8, "coriolis": -3.0}),
+        (f"burgers-periodic-{s.name}", pda.AdvectionDiffusion2d.BurgersPeriodic, s, (18, 16),
+         (-1, 1, -1, 1), 7, True, 1, {"diffusion": 1e-4}),
+    ]
+for s in (S.FirstOrder, S.Weno3):
+    CASES += [
+        (f"euler3d-sedov-{s.name}", pda.Euler3d.SedovSymmetry, s, (10, 9, 8), (0, 1.2, 0, 1.2, 0, 1.2), 5, (), 0, None),
+        (f"euler3d-periodic-{s.name}", pda.Euler3d.PeriodicSmooth, s, (8, 8, 8), (-1, 1, -1, 1, -1, 1), 5, True, 0, None),
+    ]
+
+IDS = [c[0] for c in CASES]
+
+ORACLE_PARAMS = {
+    1: lambda p: [1.4],
+    2: lambda p: [(p or {}).get("gamma", 1.4)],
+    3: lambda p: [1.4],
+    4: lambda p: [(p or {}).get("gravity", 9.8), (p or {}).get("coriolis", -3.0)],
+    5: lambda p: [(p or {}).get("diffusion", 1e-5)],
+}
+
+
+def make(case):
+    _, enum, scheme, n, bounds, st, per, ic, params = case
+    mesh = pda.create_structured_mesh(n, bounds, st, per)
+    prob = pda.create_problem(mesh, enum, scheme, ic, params)
+    return mesh, prob
+
+
+def oracle_problem(case):
+    import pyoracle
+    _, enum, scheme, n, bounds, st, per, ic, params = case
+    fam = pda.FAMILY[type(enum)]
+    return pyoracle.Oracle.problem(fam, int(enum), int(scheme), ORACLE_PARAMS[fam](params))
+
+
+def perturbed_state(prob, seed, amp=0.01):
+    """IC times (1 + U(-amp, amp)) per entry, computed on the host."""
+    import ctypes
+    U0 = np.zeros(prob.totalDofStencilMesh())
+    rc = pda.lib().pda_initial_condition_host(prob.handle, U0.ctypes.data_as(ctypes.c_void_p))
+    assert rc == 0
+    rng = np.random.default_rng(seed)
+    return U0 * (1.0 + amp * rng.uniform(-1.0, 1.0, U0.shape))
+
+
+def rel_err(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / (den if den > 0 else 1.0))
