@@ -1,0 +1,98 @@
+"""Copy the golden data files the reference's own tests hold into
+tests/golden/ref/, with a provenance manifest (source path + sha256).
+
+Run where the reference sources are present:
+    python tests/golden/collect_reference_fixtures.py [/path/to/reference]
+
+Only data is copied (meshes, gold states); trajectory golds stored as text in
+the reference are converted to .npy (float64, the values np.loadtxt parses).
+"""
+import hashlib
+import json
+import os
+import shutil
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "ref")
+
+MESH_DIRS = {
+    "tests_mesh/mesh_generation_full_1d": ["mesh_s3_5x1_nonperiodic", "mesh_s3_5x1_periodic",
+                                           "mesh_s5_5x1_nonperiodic", "mesh_s5_5x1_periodic",
+                                           "mesh_s7_5x1_nonperiodic", "mesh_s7_5x1_periodic"],
+    "tests_mesh/mesh_generation_full_2d": ["mesh_s3_5x5_periodic_x", "mesh_s3_5x5_periodic_xy",
+                                           "mesh_s3_5x5_periodic_y", "mesh_s3_5x6_nonperiodic",
+                                           "mesh_s3_5x6_periodic_xy", "mesh_s3_6x5_periodic_xy",
+                                           "mesh_s5_7x8_nonperiodic", "mesh_s5_7x8_periodic_x",
+                                           "mesh_s5_7x8_periodic_xy", "mesh_s5_8x7_nonperiodic",
+                                           "mesh_s5_8x7_periodic_xy", "mesh_s5_8x8_nonperiodic",
+                                           "mesh_s5_8x8_periodic_xy", "mesh_s7_7x8_periodic_x",
+                                           "mesh_s7_7x8_periodic_xy", "mesh_s7_8x7_periodic_xy",
+                                           "mesh_s7_8x8_periodic_xy"],
+    "tests_mesh/mesh_generation_full_3d": ["mesh_s3_5x5x5_nonperiodic", "mesh_s3_5x5x5_periodic_xyz",
+                                           "mesh_s5_5x5x5_nonperiodic", "mesh_s5_5x5x5_periodic_xyz"],
+}
+SAMPLE_DIR = "tests_mesh/mesh_generation_sample_2d/mesh_s3_5x6_nonperiodic_sample"
+
+TRAJ = [
+    "tests_cpp/eigen_1d_euler_sod_explicit/firstorder/gold.txt",
+    "tests_cpp/eigen_1d_euler_sod_explicit/weno3/gold.txt",
+    "tests_cpp/eigen_1d_euler_sod_explicit/weno5/gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_explicit/firstorder/rho_gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_explicit/firstorder/p_gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_explicit/weno3/rho_gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_explicit/weno3/p_gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_explicit/weno5/rho_gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_explicit/weno5/p_gold.txt",
+    "tests_cpp/eigen_2d_swe_slip_wall_explicit/firstorder/h_gold.txt",
+    "tests_cpp/eigen_2d_swe_slip_wall_explicit/weno3/h_gold.txt",
+    "tests_cpp/eigen_2d_swe_slip_wall_explicit/weno5/h_gold.txt",
+    "tests_cpp/eigen_2d_burgers_periodic_explicit/firstorder/gold.txt",
+    "tests_cpp/eigen_2d_burgers_periodic_explicit/weno3/gold.txt",
+    "tests_cpp/eigen_2d_burgers_periodic_explicit/weno5/gold.txt",
+    "tests_cpp/eigen_3d_euler_sedov_symmetry/firstorder_nxnynz_equal/gold_state.txt",
+    "tests_cpp/eigen_3d_euler_sedov_symmetry/weno3_nxnynz_equal/gold_state.txt",
+]
+
+
+def sha(path):
+    return hashlib.sha256(open(path, "rb").read()).hexdigest()
+
+
+def main(ref):
+    manifest = []
+    os.makedirs(OUT, exist_ok=True)
+    for base, names in MESH_DIRS.items():
+        for n in names:
+            src = os.path.join(ref, base, n)
+            dst = os.path.join(OUT, "mesh", os.path.basename(base), n)
+            os.makedirs(dst, exist_ok=True)
+            for f in ("info_gold.dat", "connectivity_gold.dat", "coordinates_gold.dat"):
+                shutil.copyfile(os.path.join(src, f), os.path.join(dst, f))
+                manifest.append({"file": os.path.relpath(os.path.join(dst, f), HERE),
+                                 "source": f"{base}/{n}/{f}", "sha256": sha(os.path.join(src, f))})
+    src = os.path.join(ref, SAMPLE_DIR)
+    dst = os.path.join(OUT, "mesh", "mesh_generation_sample_2d", os.path.basename(SAMPLE_DIR))
+    os.makedirs(dst, exist_ok=True)
+    for f in ("info.dat", "connectivity.dat", "coordinates.dat", "sample_mesh_gids.dat",
+              "stencil_mesh_gids.dat"):
+        shutil.copyfile(os.path.join(src, f), os.path.join(dst, f))
+        manifest.append({"file": os.path.relpath(os.path.join(dst, f), HERE),
+                         "source": f"{SAMPLE_DIR}/{f}", "sha256": sha(os.path.join(src, f))})
+    for rel in TRAJ:
+        src = os.path.join(ref, rel)
+        name = rel.replace("tests_cpp/", "").replace("/", "__").replace(".txt", ".npy")
+        dst = os.path.join(OUT, "traj", name)
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        np.save(dst, np.loadtxt(src))
+        manifest.append({"file": os.path.relpath(dst, HERE), "source": rel, "sha256": sha(src),
+                         "note": "np.loadtxt of the source text, saved as float64 .npy"})
+    with open(os.path.join(OUT, "MANIFEST.json"), "w") as f:
+        json.dump(manifest, f, indent=1)
+    print(f"{len(manifest)} fixtures written to {OUT}")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

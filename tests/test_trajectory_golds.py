@@ -1,0 +1,101 @@
+"""Trajectory golds the reference's own tests commit (tests_cpp/*/gold*.txt),
+reproduced with the CPU oracle's residual and the pressio integrator formulas
+(tests_cpp/pressio/include/pressio/ode/impl/ode_explicit_stepper_without_mass_matrix.hpp:230-352).
+Meshes come from the product generator (pinned by tests_mesh) and initial
+conditions from the product set-up; both are what the reference test builds."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle
+import pressiodemoapps_amd as pda
+
+TRAJ = os.path.join(os.path.dirname(__file__), "golden", "ref", "traj")
+S = pda.InviscidFluxReconstruction
+
+
+def rk4(f, y, dt, n):
+    # ode_explicit_stepper_without_mass_matrix.hpp:284-352
+    for _ in range(n):
+        k1 = f(y)
+        k2 = f(y + (dt / 2) * k1)
+        k3 = f(y + (dt / 2) * k2)
+        k4 = f(y + dt * k3)
+        y = y + (dt / 6) * k1 + (dt / 3) * k2 + (dt / 3) * k3 + (dt / 6) * k4
+    return y
+
+
+def ssprk3(f, y, dt, n):
+    # ode_explicit_stepper_without_mass_matrix.hpp:230-281
+    for _ in range(n):
+        a = y + dt * f(y)
+        a = 0.25 * a + 0.75 * y + (0.25 * dt) * f(a)
+        y = (1. / 3.) * y + (2. / 3.) * a + (2. / 3. * dt) * f(a)
+    return y
+
+
+def _setup(enum, scheme, n, bounds, st, per=(), ic=0, params=None, oparams=(1.4,)):
+    mesh = pda.create_structured_mesh(n, bounds, st, per)
+    prob = pda.create_problem(mesh, enum, scheme, ic, params)
+    U0 = np.zeros(prob.totalDofStencilMesh())
+    assert pda.lib().pda_initial_condition_host(prob.handle, U0.ctypes.data_as(ctypes.c_void_p)) == 0
+    orc = pyoracle.Oracle()
+    ms = orc.from_mesh(mesh)
+    pb = orc.problem(pda.FAMILY[type(enum)], int(enum), int(scheme), list(oparams))
+    return U0, (lambda U: orc.velocity(ms, pb, U))
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3"),
+                                            (S.Weno5, 7, "weno5")])
+def test_sod1d_rk4(scheme, st, name):
+    # tests_cpp/eigen_1d_euler_sod_explicit: 100 cells on [-0.5,0.5], RK4 dt 1e-3 x 100
+    U0, f = _setup(pda.Euler1d.Sod, scheme, (100,), (-0.5, 0.5), st)
+    U = rk4(f, U0, 1e-3, 100)
+    gold = np.load(os.path.join(TRAJ, f"eigen_1d_euler_sod_explicit__{name}__gold.npy"))
+    assert np.allclose(U, gold, rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 7, "weno3"),
+                                            (S.Weno5, 7, "weno5")])
+def test_riemann2d_ssprk3(scheme, st, name):
+    # tests_cpp/eigen_2d_euler_riemann_explicit: 20x20 on [0,1]^2, IC2, SSPRK3 dt 0.01 x 60
+    U0, f = _setup(pda.Euler2d.Riemann, scheme, (20, 20), (0.0, 1.0, 0.0, 1.0), st, ic=2)
+    U = ssprk3(f, U0, 0.01, 60).reshape(-1, 4)
+    rho = U[:, 0]
+    p = 0.4 * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / rho)
+    base = f"eigen_2d_euler_riemann_explicit__{name}__"
+    assert np.allclose(rho, np.load(os.path.join(TRAJ, base + "rho_gold.npy")), rtol=1e-10, atol=1e-12)
+    assert np.allclose(p, np.load(os.path.join(TRAJ, base + "p_gold.npy")), rtol=1e-9, atol=1e-11)
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3"),
+                                            (S.Weno5, 7, "weno5")])
+def test_swe_slipwall_rk4(scheme, st, name):
+    # tests_cpp/eigen_2d_swe_slip_wall_explicit: 25x25 on [-5,5]^2, RK4 dt 0.01 x 200
+    U0, f = _setup(pda.Swe2d.SlipWall, scheme, (25, 25), (-5.0, 5.0, -5.0, 5.0), st,
+                   oparams=(9.8, -3.0))
+    U = rk4(f, U0, 0.01, 200).reshape(-1, 3)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_swe_slip_wall_explicit__{name}__h_gold.npy"))
+    assert np.allclose(U[:, 0], gold, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3"),
+                                            (S.Weno5, 7, "weno5")])
+def test_burgers_periodic_rk4(scheme, st, name):
+    # tests_cpp/eigen_2d_burgers_periodic_explicit: 20x20 on [-1,1]^2 periodic, RK4 dt 0.01 x 200
+    U0, f = _setup(pda.AdvectionDiffusion2d.BurgersPeriodic, scheme, (20, 20), (-1.0, 1.0, -1.0, 1.0),
+                   st, per=True, oparams=(1e-5,))
+    U = rk4(f, U0, 0.01, 200)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_burgers_periodic_explicit__{name}__gold.npy"))
+    assert np.allclose(U, gold, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3")])
+def test_sedov3d_ssprk3(scheme, st, name):
+    # tests_cpp/eigen_3d_euler_sedov_symmetry: 20^3 on [0,0.4]^3, SSPRK3 dt 1e-4 x 250
+    U0, f = _setup(pda.Euler3d.SedovSymmetry, scheme, (20, 20, 20), (0.0, 0.4, 0.0, 0.4, 0.0, 0.4), st)
+    U = ssprk3(f, U0, 1e-4, 250)
+    gold = np.load(os.path.join(TRAJ, f"eigen_3d_euler_sedov_symmetry__{name}_nxnynz_equal__gold_state.npy"))
+    assert np.allclose(U, gold, rtol=1e-8, atol=1e-10)
