@@ -27,7 +27,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["pda_mesh.cpp", "pda_problem.cpp", "pda_kernels.hip"]
+SOURCES = ["pda_mesh.cpp", "pda_problem.cpp", "pda_kernels.hip", "pda_box_kernels.hip"]
 
 
 def _compile(src):

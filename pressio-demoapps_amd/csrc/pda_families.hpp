@@ -1,0 +1,230 @@
+// pda_families.hpp -- per-family policies used by every kernel: dofs per
+// cell, dimension, face flux and flux Jacobians along axis A, and the source
+// terms applied after the axis sweep; plus the stencil helpers.
+#pragma once
+#include "pda_internal.hpp"
+#include "pda_physics.hpp"
+
+namespace pda {
+namespace fam {
+
+// ------------------------------------------------------------------ physics
+// Each family: M dofs per cell, DIM axes, face flux and its Jacobians for
+// axis A, and the per-cell source terms applied after the axis sweep.
+struct PhEuler1 {
+  static constexpr int M = 3, DIM = 1;
+  template <int A>
+  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
+    phys::euler_flux<1, A>(l, r, P.p[0], F);
+  }
+  template <int A>
+  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
+    phys::euler_fjac<1, A>(l, r, P.p[0], JL, JR);
+  }
+};
+struct PhEuler2 {
+  static constexpr int M = 4, DIM = 2;
+  template <int A>
+  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
+    phys::euler_flux<2, A>(l, r, P.p[0], F);
+  }
+  template <int A>
+  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
+    phys::euler_fjac<2, A>(l, r, P.p[0], JL, JR);
+  }
+};
+struct PhEuler3 {
+  static constexpr int M = 5, DIM = 3;
+  template <int A>
+  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
+    phys::euler_flux<3, A>(l, r, P.p[0], F);
+  }
+  template <int A>
+  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
+    phys::euler_fjac<3, A>(l, r, P.p[0], JL, JR);
+  }
+};
+struct PhSwe {
+  static constexpr int M = 3, DIM = 2;
+  template <int A>
+  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
+    phys::swe_flux<A>(l, r, P.p[0], F);
+  }
+  template <int A>
+  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
+    phys::swe_fjac<A>(l, r, P.p[0], JL, JR);
+  }
+};
+struct PhBurgers {
+  static constexpr int M = 2, DIM = 2;
+  template <int A>
+  __device__ static void flux(const View &, const double *l, const double *r, double *F) {
+    phys::burgers_flux<A>(l, r, F);
+  }
+  template <int A>
+  __device__ static void fjac(const View &, const double *l, const double *r, double *JL, double *JR) {
+    phys::burgers_fjac<A>(l, r, JL, JR);
+  }
+};
+
+template <class Ph> struct Sources {
+  // V sources after the axis sweep; the Jacobian part updates the diagonal
+  // block held in registers (self) and returns the diagonal addend of
+  // first-neighbour blocks through offdiag()
+  __device__ static void apply(const View &, int64_t, bool, const double *, int32_t,
+                               double *, double *) {}
+  __device__ static double offdiag(const View &, bool, int, int, int) { return 0.; }
+};
+
+// Coriolis forcing, impl/swe_2d_prob_class.hpp:984-1012
+template <> struct Sources<PhSwe> {
+  __device__ static void apply(const View &P, int64_t, bool, const double *U, int32_t self,
+                               double *v, double *J) {
+    const double f = P.p[1];
+    const double h = U[3 * int64_t(self)], hu = U[3 * int64_t(self) + 1], hv = U[3 * int64_t(self) + 2];
+    v[1] -= f * hv / h;
+    v[2] += f * hu / h;
+    if (J) {
+      J[1 * 3 + 0] += f * hv / (h * h);
+      J[1 * 3 + 2] += -f / h;
+      J[2 * 3 + 1] += f / h;
+      J[2 * 3 + 0] += -f * hu / (h * h);
+    }
+  }
+  __device__ static double offdiag(const View &, bool, int, int, int) { return 0.; }
+};
+
+// Burgers viscous term on inner cells,
+// impl/advection_diffusion_2d_prob_class.hpp:1167-1204
+template <> struct Sources<PhBurgers> {
+  __device__ static void apply(const View &P, int64_t r, bool near, const double *U, int32_t self,
+                               double *v, double *J) {
+    if (near) return;
+    const int32_t *gr = P.graph + r * P.ncols;
+    const double dX = P.p[0] * (P.hinv[0] * P.hinv[0]);
+    const double dY = P.p[0] * (P.hinv[1] * P.hinv[1]);
+    const int64_t ui = 2 * int64_t(self), uL = 2 * int64_t(gr[1]), uF = 2 * int64_t(gr[2]);
+    const int64_t uR = 2 * int64_t(gr[3]), uB = 2 * int64_t(gr[4]);
+    for (int k = 0; k < 2; ++k) {
+      v[k] += dX * (U[uR + k] - 2. * U[ui + k] + U[uL + k]);
+      v[k] += dY * (U[uF + k] - 2. * U[ui + k] + U[uB + k]);
+    }
+    if (J) {
+      J[0] += -2. * dX - 2. * dY;
+      J[3] += -2. * dX - 2. * dY;
+    }
+  }
+  // the diffusion adds to the diagonal of the four first-neighbour blocks
+  __device__ static double offdiag(const View &P, bool near, int a, int k, int j) {
+    if (near || k != j) return 0.;
+    return P.p[0] * (P.hinv[a] * P.hinv[a]);
+  }
+};
+
+// ------------------------------------------------------------------ helpers
+template <int DIM>
+__device__ __forceinline__ int colof(int a, int side, int L) {
+  if (DIM == 1) return 1 + 2 * L + side;
+  if (DIM == 2) return a == 0 ? 1 + 4 * L + (side ? 2 : 0) : 1 + 4 * L + (side ? 1 : 3);
+  const int base = 1 + 6 * L;
+  return a == 0 ? base + (side ? 2 : 0) : a == 1 ? base + (side ? 1 : 3) : base + (side ? 5 : 4);
+}
+
+__device__ __forceinline__ const double *ghost_ptr(const View &P, int a, int side, int64_t g,
+                                                   int wm) {
+  return P.ghost + ((int64_t(a) * 2 + side) * P.n_near + g) * wm;
+}
+
+// st[p][k], p = 0..2W, centre at W.  Missing neighbours (graph -1) of a
+// near-boundary row are read from its ghost layers, as
+// impl/functor_fill_stencil.hpp does.  GW is the ghost-layer width the ghost
+// arrays were filled with (the scheme's), W the width gathered.
+template <class Ph, int W, int A>
+__device__ __forceinline__ void gather(const View &P, const int32_t *gr, int64_t g, int GW,
+                                       const double *__restrict__ U, double st[][Ph::M]) {
+  constexpr int M = Ph::M;
+  const int64_t self = gr[0];
+#pragma unroll
+  for (int k = 0; k < M; ++k) st[W][k] = U[self * M + k];
+#pragma unroll
+  for (int side = 0; side < 2; ++side)
+#pragma unroll
+    for (int L = 0; L < W; ++L) {
+      const int32_t nb = gr[colof<Ph::DIM>(A, side, L)];
+      const int p = side ? W + 1 + L : W - 1 - L;
+      if (nb >= 0) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) st[p][k] = U[int64_t(nb) * M + k];
+      } else {
+        const double *gp = ghost_ptr(P, A, side, g, GW * M) + L * M;
+#pragma unroll
+        for (int k = 0; k < M; ++k) st[p][k] = gp[k];
+      }
+    }
+}
+
+// face states of the cell's left (minus) and right (plus) faces
+template <int M, int W>
+struct Faces {
+  double lm[M], lp[M], rm[M], rp[M];
+};
+template <int M, int W>
+struct FaceGrads {
+  double glm[M][2 * W], glp[M][2 * W], grm[M][2 * W], grp[M][2 * W];
+};
+
+template <int M, int W>
+__device__ __forceinline__ void reconstruct(const double st[][M], Faces<M, W> &f) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    double q[2 * W + 1];
+#pragma unroll
+    for (int p = 0; p <= 2 * W; ++p) q[p] = st[p][k];
+    if constexpr (W == 1) {
+      f.lm[k] = q[0]; f.lp[k] = q[1]; f.rm[k] = q[1]; f.rp[k] = q[2];
+    } else if constexpr (W == 2) {
+      phys::weno3(q, f.lm[k], f.lp[k]);
+      phys::weno3(q + 1, f.rm[k], f.rp[k]);
+    } else {
+      phys::weno5(q, f.lm[k], f.lp[k]);
+      phys::weno5(q + 1, f.rm[k], f.rp[k]);
+    }
+  }
+}
+
+template <int M, int W>
+__device__ __forceinline__ void reconstruct_grad(const double st[][M], Faces<M, W> &f,
+                                                 FaceGrads<M, W> &G) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    double q[2 * W + 1];
+#pragma unroll
+    for (int p = 0; p <= 2 * W; ++p) q[p] = st[p][k];
+    if constexpr (W == 2) {
+      phys::weno3_grad(q, f.lm[k], f.lp[k], G.glm[k], G.glp[k]);
+      phys::weno3_grad(q + 1, f.rm[k], f.rp[k], G.grm[k], G.grp[k]);
+    } else {
+      phys::weno5_grad(q, f.lm[k], f.lp[k], G.glm[k], G.glp[k]);
+      phys::weno5_grad(q + 1, f.rm[k], f.rp[k], G.grm[k], G.grp[k]);
+    }
+  }
+}
+
+// store (or, when a row maps two stencil columns onto one block, add) one
+// m x m block at a slot of the row
+template <int M>
+__device__ __forceinline__ void put_block(const View &P, double *__restrict__ J, int64_t base,
+                                          int rs, int s, const double (&b)[M][M]) {
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      double *dst = J + base + int64_t(k) * rs + s * M + j;
+      if (P.dup) *dst += b[k][j];
+      else *dst = b[k][j];
+    }
+}
+
+
+}  // namespace fam
+}  // namespace pda

@@ -121,6 +121,21 @@ struct View {
   double hinv[3] = {0, 0, 0};
   double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double gsign[3][2][5];              // ghost value sign per axis, side, dof
+  // rows the generic kernels visit: all rows when rows == nullptr
+  const int32_t *rows = nullptr;
+  int64_t nvisit = 0;
+};
+
+// Index box of a natural-order structured mesh whose cells are inner rows
+// with their whole scheme stencil inside the grid (no periodic wrap): the
+// structured kernels own these rows, the generic kernels the rest.
+struct Box {
+  int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}; // half-open, per axis
+  int64_t n[3] = {1, 1, 1};                     // grid extent
+  bool valid = false;
+  int64_t cells() const {
+    return valid ? (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) : 0;
+  }
 };
 
 enum GhostRule { GHOST_NONE = 0, GHOST_OPPOSITE = 1, GHOST_MIRROR = 2 };
@@ -144,6 +159,11 @@ struct pda_problem_s {
   std::vector<uint8_t> h_slot;
   std::vector<double> h_bcf;
   bool on_device = false;
+  // structured fast path
+  pda::Box box;
+  std::vector<int32_t> h_rem_rows;   // rows outside the box
+  pda::DevBuf d_rem_rows;
+  int use_box = 1;                   // 0 forces the generic kernels everywhere
   pda::View view;
   pda::DevBuf d_graph, d_ghost_row, d_near_rows, d_ghost, d_bcf;
   pda::DevBuf d_rowptr, d_colidx, d_rowbase, d_nblk, d_slot, d_bcol;
@@ -158,4 +178,10 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double t, double *J
                     double *V, hipStream_t s);
 int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64_t ncols,
                  int layout, double *AJ, hipStream_t s);
+// structured-box kernels; return PDA_E_UNSUPPORTED when the problem has no
+// box kernel (the caller then covers all rows with the generic kernels)
+int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s);
+int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, double *V,
+                        hipStream_t s);
+bool has_box_kernels(const pda_problem_s &p);
 }  // namespace pda

@@ -13,228 +13,11 @@
 //     (impl/mixin_directional_flux_balance_jacobian.hpp:142-284);
 //   - near-boundary rows: first-order blocks with BC factors folded into the
 //     diagonal block (impl/mixin_directional_flux_balance_jacobian.hpp:311-371).
-#include "pda_internal.hpp"
-#include "pda_physics.hpp"
+#include "pda_families.hpp"
 
 namespace pda {
 namespace {
-
-// ------------------------------------------------------------------ physics
-// Each family: M dofs per cell, DIM axes, face flux and its Jacobians for
-// axis A, and the per-cell source terms applied after the axis sweep.
-struct PhEuler1 {
-  static constexpr int M = 3, DIM = 1;
-  template <int A>
-  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::euler_flux<1, A>(l, r, P.p[0], F);
-  }
-  template <int A>
-  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::euler_fjac<1, A>(l, r, P.p[0], JL, JR);
-  }
-};
-struct PhEuler2 {
-  static constexpr int M = 4, DIM = 2;
-  template <int A>
-  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::euler_flux<2, A>(l, r, P.p[0], F);
-  }
-  template <int A>
-  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::euler_fjac<2, A>(l, r, P.p[0], JL, JR);
-  }
-};
-struct PhEuler3 {
-  static constexpr int M = 5, DIM = 3;
-  template <int A>
-  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::euler_flux<3, A>(l, r, P.p[0], F);
-  }
-  template <int A>
-  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::euler_fjac<3, A>(l, r, P.p[0], JL, JR);
-  }
-};
-struct PhSwe {
-  static constexpr int M = 3, DIM = 2;
-  template <int A>
-  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::swe_flux<A>(l, r, P.p[0], F);
-  }
-  template <int A>
-  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::swe_fjac<A>(l, r, P.p[0], JL, JR);
-  }
-};
-struct PhBurgers {
-  static constexpr int M = 2, DIM = 2;
-  template <int A>
-  __device__ static void flux(const View &, const double *l, const double *r, double *F) {
-    phys::burgers_flux<A>(l, r, F);
-  }
-  template <int A>
-  __device__ static void fjac(const View &, const double *l, const double *r, double *JL, double *JR) {
-    phys::burgers_fjac<A>(l, r, JL, JR);
-  }
-};
-
-template <class Ph> struct Sources {
-  // V sources after the axis sweep; the Jacobian part updates the diagonal
-  // block held in registers (self) and returns the diagonal addend of
-  // first-neighbour blocks through offdiag()
-  __device__ static void apply(const View &, int64_t, bool, const double *, int32_t,
-                               double *, double *) {}
-  __device__ static double offdiag(const View &, bool, int, int, int) { return 0.; }
-};
-
-// Coriolis forcing, impl/swe_2d_prob_class.hpp:984-1012
-template <> struct Sources<PhSwe> {
-  __device__ static void apply(const View &P, int64_t, bool, const double *U, int32_t self,
-                               double *v, double *J) {
-    const double f = P.p[1];
-    const double h = U[3 * int64_t(self)], hu = U[3 * int64_t(self) + 1], hv = U[3 * int64_t(self) + 2];
-    v[1] -= f * hv / h;
-    v[2] += f * hu / h;
-    if (J) {
-      J[1 * 3 + 0] += f * hv / (h * h);
-      J[1 * 3 + 2] += -f / h;
-      J[2 * 3 + 1] += f / h;
-      J[2 * 3 + 0] += -f * hu / (h * h);
-    }
-  }
-  __device__ static double offdiag(const View &, bool, int, int, int) { return 0.; }
-};
-
-// Burgers viscous term on inner cells,
-// impl/advection_diffusion_2d_prob_class.hpp:1167-1204
-template <> struct Sources<PhBurgers> {
-  __device__ static void apply(const View &P, int64_t r, bool near, const double *U, int32_t self,
-                               double *v, double *J) {
-    if (near) return;
-    const int32_t *gr = P.graph + r * P.ncols;
-    const double dX = P.p[0] * (P.hinv[0] * P.hinv[0]);
-    const double dY = P.p[0] * (P.hinv[1] * P.hinv[1]);
-    const int64_t ui = 2 * int64_t(self), uL = 2 * int64_t(gr[1]), uF = 2 * int64_t(gr[2]);
-    const int64_t uR = 2 * int64_t(gr[3]), uB = 2 * int64_t(gr[4]);
-    for (int k = 0; k < 2; ++k) {
-      v[k] += dX * (U[uR + k] - 2. * U[ui + k] + U[uL + k]);
-      v[k] += dY * (U[uF + k] - 2. * U[ui + k] + U[uB + k]);
-    }
-    if (J) {
-      J[0] += -2. * dX - 2. * dY;
-      J[3] += -2. * dX - 2. * dY;
-    }
-  }
-  // the diffusion adds to the diagonal of the four first-neighbour blocks
-  __device__ static double offdiag(const View &P, bool near, int a, int k, int j) {
-    if (near || k != j) return 0.;
-    return P.p[0] * (P.hinv[a] * P.hinv[a]);
-  }
-};
-
-// ------------------------------------------------------------------ helpers
-template <int DIM>
-__device__ __forceinline__ int colof(int a, int side, int L) {
-  if (DIM == 1) return 1 + 2 * L + side;
-  if (DIM == 2) return a == 0 ? 1 + 4 * L + (side ? 2 : 0) : 1 + 4 * L + (side ? 1 : 3);
-  const int base = 1 + 6 * L;
-  return a == 0 ? base + (side ? 2 : 0) : a == 1 ? base + (side ? 1 : 3) : base + (side ? 5 : 4);
-}
-
-__device__ __forceinline__ const double *ghost_ptr(const View &P, int a, int side, int64_t g,
-                                                   int wm) {
-  return P.ghost + ((int64_t(a) * 2 + side) * P.n_near + g) * wm;
-}
-
-// st[p][k], p = 0..2W, centre at W.  Missing neighbours (graph -1) of a
-// near-boundary row are read from its ghost layers, as
-// impl/functor_fill_stencil.hpp does.  GW is the ghost-layer width the ghost
-// arrays were filled with (the scheme's), W the width gathered.
-template <class Ph, int W, int A>
-__device__ __forceinline__ void gather(const View &P, const int32_t *gr, int64_t g, int GW,
-                                       const double *__restrict__ U, double st[][Ph::M]) {
-  constexpr int M = Ph::M;
-  const int64_t self = gr[0];
-#pragma unroll
-  for (int k = 0; k < M; ++k) st[W][k] = U[self * M + k];
-#pragma unroll
-  for (int side = 0; side < 2; ++side)
-#pragma unroll
-    for (int L = 0; L < W; ++L) {
-      const int32_t nb = gr[colof<Ph::DIM>(A, side, L)];
-      const int p = side ? W + 1 + L : W - 1 - L;
-      if (nb >= 0) {
-#pragma unroll
-        for (int k = 0; k < M; ++k) st[p][k] = U[int64_t(nb) * M + k];
-      } else {
-        const double *gp = ghost_ptr(P, A, side, g, GW * M) + L * M;
-#pragma unroll
-        for (int k = 0; k < M; ++k) st[p][k] = gp[k];
-      }
-    }
-}
-
-// face states of the cell's left (minus) and right (plus) faces
-template <int M, int W>
-struct Faces {
-  double lm[M], lp[M], rm[M], rp[M];
-};
-template <int M, int W>
-struct FaceGrads {
-  double glm[M][2 * W], glp[M][2 * W], grm[M][2 * W], grp[M][2 * W];
-};
-
-template <int M, int W>
-__device__ __forceinline__ void reconstruct(const double st[][M], Faces<M, W> &f) {
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    double q[2 * W + 1];
-#pragma unroll
-    for (int p = 0; p <= 2 * W; ++p) q[p] = st[p][k];
-    if constexpr (W == 1) {
-      f.lm[k] = q[0]; f.lp[k] = q[1]; f.rm[k] = q[1]; f.rp[k] = q[2];
-    } else if constexpr (W == 2) {
-      phys::weno3(q, f.lm[k], f.lp[k]);
-      phys::weno3(q + 1, f.rm[k], f.rp[k]);
-    } else {
-      phys::weno5(q, f.lm[k], f.lp[k]);
-      phys::weno5(q + 1, f.rm[k], f.rp[k]);
-    }
-  }
-}
-
-template <int M, int W>
-__device__ __forceinline__ void reconstruct_grad(const double st[][M], Faces<M, W> &f,
-                                                 FaceGrads<M, W> &G) {
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    double q[2 * W + 1];
-#pragma unroll
-    for (int p = 0; p <= 2 * W; ++p) q[p] = st[p][k];
-    if constexpr (W == 2) {
-      phys::weno3_grad(q, f.lm[k], f.lp[k], G.glm[k], G.glp[k]);
-      phys::weno3_grad(q + 1, f.rm[k], f.rp[k], G.grm[k], G.grp[k]);
-    } else {
-      phys::weno5_grad(q, f.lm[k], f.lp[k], G.glm[k], G.glp[k]);
-      phys::weno5_grad(q + 1, f.rm[k], f.rp[k], G.grm[k], G.grp[k]);
-    }
-  }
-}
-
-// store (or, when a row maps two stencil columns onto one block, add) one
-// m x m block at a slot of the row
-template <int M>
-__device__ __forceinline__ void put_block(const View &P, double *__restrict__ J, int64_t base,
-                                          int rs, int s, const double (&b)[M][M]) {
-#pragma unroll
-  for (int k = 0; k < M; ++k)
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      double *dst = J + base + int64_t(k) * rs + s * M + j;
-      if (P.dup) *dst += b[k][j];
-      else *dst = b[k][j];
-    }
-}
+using namespace fam;
 
 // ---------------------------------------------------------------- ghosts
 // Ghost layers of every near-boundary row.  Layer 0 copies the cell itself;
@@ -307,8 +90,9 @@ template <class Ph, int W>
 __global__ void __launch_bounds__(256) k_velocity(View P, const double *__restrict__ U,
                                                    double *__restrict__ V) {
   constexpr int M = Ph::M;
-  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= P.nrows) return;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= P.nvisit) return;
+  const int64_t r = P.rows ? P.rows[t] : t;
   const int32_t *gr = P.graph + r * P.ncols;
   const int64_t g = P.ghost_row[r];
   double acc[M];
@@ -356,6 +140,7 @@ __device__ __forceinline__ void jac_axis_inner(const View &P, const int32_t *gr,
     put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 0, 0)], bl);
     put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 1, 0)], br);
   } else {
+#pragma clang fp contract(off)
     FaceGrads<M, W> G;
     reconstruct_grad<M, W>(st, f, G);
     Ph::template flux<A>(P, f.lm, f.lp, FL);
@@ -469,8 +254,9 @@ template <class Ph, int W>
 __global__ void __launch_bounds__(256) k_jacobian(View P, const double *__restrict__ U,
                                                    double *__restrict__ V, double *__restrict__ J) {
   constexpr int M = Ph::M;
-  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= P.nrows) return;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= P.nvisit) return;
+  const int64_t r = P.rows ? P.rows[t] : t;
   const int32_t *gr = P.graph + r * P.ncols;
   const int64_t g = P.ghost_row[r];
   const uint8_t *sl = P.slot + r * P.npos;
@@ -539,38 +325,67 @@ int launch_ghosts(const pda_problem_s &p, const double *U, hipStream_t s) {
 }
 
 template <class Ph>
-static int vel_dispatch(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
+static int vel_dispatch(const View &v, int w, const double *U, double *V, hipStream_t s) {
   const int bs = 256;
-  const unsigned nb = unsigned((p.view.nrows + bs - 1) / bs);
-  if (p.w == 1) hipLaunchKernelGGL((k_velocity<Ph, 1>), dim3(nb), dim3(bs), 0, s, p.view, U, V);
-  else if (p.w == 2) hipLaunchKernelGGL((k_velocity<Ph, 2>), dim3(nb), dim3(bs), 0, s, p.view, U, V);
-  else hipLaunchKernelGGL((k_velocity<Ph, 3>), dim3(nb), dim3(bs), 0, s, p.view, U, V);
+  if (v.nvisit == 0) return PDA_OK;
+  const unsigned nb = unsigned((v.nvisit + bs - 1) / bs);
+  if (w == 1) hipLaunchKernelGGL((k_velocity<Ph, 1>), dim3(nb), dim3(bs), 0, s, v, U, V);
+  else if (w == 2) hipLaunchKernelGGL((k_velocity<Ph, 2>), dim3(nb), dim3(bs), 0, s, v, U, V);
+  else hipLaunchKernelGGL((k_velocity<Ph, 3>), dim3(nb), dim3(bs), 0, s, v, U, V);
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
 
 template <class Ph>
-static int jac_dispatch(const pda_problem_s &p, const double *U, double *V, double *J,
+static int jac_dispatch(const View &v, int w, const double *U, double *V, double *J,
                         hipStream_t s) {
   const int bs = 256;
-  const unsigned nb = unsigned((p.view.nrows + bs - 1) / bs);
-  if (p.w == 1) hipLaunchKernelGGL((k_jacobian<Ph, 1>), dim3(nb), dim3(bs), 0, s, p.view, U, V, J);
-  else if (p.w == 2) hipLaunchKernelGGL((k_jacobian<Ph, 2>), dim3(nb), dim3(bs), 0, s, p.view, U, V, J);
-  else hipLaunchKernelGGL((k_jacobian<Ph, 3>), dim3(nb), dim3(bs), 0, s, p.view, U, V, J);
+  if (v.nvisit == 0) return PDA_OK;
+  const unsigned nb = unsigned((v.nvisit + bs - 1) / bs);
+  if (w == 1) hipLaunchKernelGGL((k_jacobian<Ph, 1>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+  else if (w == 2) hipLaunchKernelGGL((k_jacobian<Ph, 2>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+  else hipLaunchKernelGGL((k_jacobian<Ph, 3>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
+}
+
+// zero fill as a kernel (a memset node would not be safe inside captured graphs)
+__global__ void k_fill(double *x, int64_t n, double v) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = v;
+}
+
+// the rows the generic kernels cover: all of them, or those outside the box
+static View generic_view(const pda_problem_s &p, bool box_done) {
+  View v = p.view;
+  if (box_done) {
+    v.rows = p.d_rem_rows.as<int32_t>();
+    v.nvisit = int64_t(p.h_rem_rows.size());
+  } else {
+    v.rows = nullptr;
+    v.nvisit = v.nrows;
+  }
+  return v;
 }
 
 int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, hipStream_t s) {
   if (p.view.nrows == 0) return PDA_OK;
   int rc = launch_ghosts(p, U, s);
   if (rc) return rc;
+  bool box_done = false;
+  if (p.use_box && p.box.valid && has_box_kernels(p)) {
+    rc = launch_velocity_box(p, U, V, s);
+    if (rc == PDA_OK) box_done = true;
+    else if (rc != PDA_E_UNSUPPORTED) return rc;
+  }
+  const View v = generic_view(p, box_done);
   switch (p.family) {
-    case PDA_EULER1D: return vel_dispatch<PhEuler1>(p, U, V, s);
-    case PDA_EULER2D: return vel_dispatch<PhEuler2>(p, U, V, s);
-    case PDA_EULER3D: return vel_dispatch<PhEuler3>(p, U, V, s);
-    case PDA_SWE2D: return vel_dispatch<PhSwe>(p, U, V, s);
-    case PDA_BURGERS2D: return vel_dispatch<PhBurgers>(p, U, V, s);
+    case PDA_EULER1D: return vel_dispatch<PhEuler1>(v, p.w, U, V, s);
+    case PDA_EULER2D: return vel_dispatch<PhEuler2>(v, p.w, U, V, s);
+    case PDA_EULER3D: return vel_dispatch<PhEuler3>(v, p.w, U, V, s);
+    case PDA_SWE2D: return vel_dispatch<PhSwe>(v, p.w, U, V, s);
+    case PDA_BURGERS2D: return vel_dispatch<PhBurgers>(v, p.w, U, V, s);
     default: set_error("velocity: unsupported family"); return PDA_E_UNSUPPORTED;
   }
 }
@@ -580,13 +395,23 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
   if (p.view.nrows == 0) return PDA_OK;
   int rc = launch_ghosts(p, U, s);
   if (rc) return rc;
-  if (p.view.dup) PDA_HIP_CHECK(hipMemsetAsync(J, 0, sizeof(double) * p.nnz, s));
+  if (p.view.dup) {
+    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, J, p.nnz, 0.);
+    PDA_HIP_CHECK(hipGetLastError());
+  }
+  bool box_done = false;
+  if (p.use_box && p.box.valid && !p.view.dup && has_box_kernels(p)) {
+    rc = launch_jacobian_box(p, U, J, V, s);
+    if (rc == PDA_OK) box_done = true;
+    else if (rc != PDA_E_UNSUPPORTED) return rc;
+  }
+  const View v = generic_view(p, box_done);
   switch (p.family) {
-    case PDA_EULER1D: return jac_dispatch<PhEuler1>(p, U, V, J, s);
-    case PDA_EULER2D: return jac_dispatch<PhEuler2>(p, U, V, J, s);
-    case PDA_EULER3D: return jac_dispatch<PhEuler3>(p, U, V, J, s);
-    case PDA_SWE2D: return jac_dispatch<PhSwe>(p, U, V, J, s);
-    case PDA_BURGERS2D: return jac_dispatch<PhBurgers>(p, U, V, J, s);
+    case PDA_EULER1D: return jac_dispatch<PhEuler1>(v, p.w, U, V, J, s);
+    case PDA_EULER2D: return jac_dispatch<PhEuler2>(v, p.w, U, V, J, s);
+    case PDA_EULER3D: return jac_dispatch<PhEuler3>(v, p.w, U, V, J, s);
+    case PDA_SWE2D: return jac_dispatch<PhSwe>(v, p.w, U, V, J, s);
+    case PDA_BURGERS2D: return jac_dispatch<PhBurgers>(v, p.w, U, V, J, s);
     default: set_error("jacobian: unsupported family"); return PDA_E_UNSUPPORTED;
   }
 }

@@ -41,6 +41,7 @@ __device__ __forceinline__ void weno3(const double *q, double &un, double &up) {
 // impl/weno3.hpp:134-246: values plus d(face)/d(q_j), j = 0..3
 __device__ __forceinline__ void weno3_grad(const double *q, double &un, double &up,
                                            double *gn, double *gp) {
+#pragma clang fp contract(off)
   constexpr double eps = 1.e-6, c13 = 1. / 3., c23 = 2. / 3.;
   {
     const double qim1 = q[0], qi = q[1], qip1 = q[2];
@@ -139,24 +140,38 @@ __device__ __forceinline__ void weno5(const double *q, double &un, double &up) {
 // one side of impl/weno5.hpp:180-434.  c = linear weights {d0,d1,d2};
 // P[k][i] = d p_k / d q_i; T/S = the two smoothness building blocks with
 // their gradients; returns value, writes grad[0..4]
+// cube with one rounding in almost all cases (the reference uses pow(x, 3.))
+__device__ __forceinline__ double cube(double e) {
+  const double s = e * e;
+  const double s_lo = __fma_rn(e, e, -s);
+  return __fma_rn(s, e, s_lo * e);
+}
+
+// The gradient terms cancel strongly where the smoothness indicators are
+// small (d alpha / d beta ~ 1/(eps+beta)^3), so this path keeps the
+// reference's expression forms and is compiled without FMA contraction.
+// dd = {1, 6, 3} or {3, 6, 1}: alpha_k = (dd_k/10) / (eps+beta_k)^2 and
+// d alpha_k / d beta_k = -dd_k / (5 (eps+beta_k)^3), impl/weno5.hpp:313-323.
 __device__ __forceinline__ double weno5_side_grad(const double p[3], const double B[3],
                                                   const double dB[3][5], const double dp[3][5],
-                                                  const double c[3], double *grad) {
+                                                  const double c[3], const double dd[3],
+                                                  double *grad) {
+#pragma clang fp contract(off)
   constexpr double eps = 1.e-6;
   double a[3], e3[3];
   for (int k = 0; k < 3; ++k) {
     const double e = eps + B[k];
     a[k] = c[k] / (e * e);
-    e3[k] = e * e * e;
+    e3[k] = cube(B[k] + eps);
   }
   const double s = 1. / (a[0] + a[1] + a[2]);
   const double S = a[2] + a[1] + a[0];
   const double mS2 = -1. / (S * S);
   const double w0 = a[0] * s, w1 = a[1] * s, w2 = a[2] * s;
   for (int i = 0; i < 5; ++i) {
-    const double da0 = -2. * c[0] / e3[0] * dB[0][i];
-    const double da1 = -2. * c[1] / e3[1] * dB[1][i];
-    const double da2 = -2. * c[2] / e3[2] * dB[2][i];
+    const double da0 = -dd[0] / (5. * e3[0]) * dB[0][i];
+    const double da1 = -dd[1] / (5. * e3[1]) * dB[1][i];
+    const double da2 = -dd[2] / (5. * e3[2]) * dB[2][i];
     const double ds = mS2 * (da0 + da1 + da2);
     double v = (da0 * s + ds * a[0]) * p[0] + dp[0][i] * w0;
     v += (da1 * s + ds * a[1]) * p[1] + dp[1][i] * w1;
@@ -169,6 +184,7 @@ __device__ __forceinline__ double weno5_side_grad(const double p[3], const doubl
 // values plus d(face)/d(q_j), j = 0..5 (gn[5] = gp[0] = 0)
 __device__ __forceinline__ void weno5_grad(const double *q, double &un, double &up,
                                            double *gn, double *gp) {
+#pragma clang fp contract(off)
   constexpr double c1312 = 13. / 12., c16 = 1. / 6.;
   {
     const double qim2 = q[0], qim1 = q[1], qi = q[2], qip1 = q[3], qip2 = q[4];
@@ -188,8 +204,8 @@ __device__ __forceinline__ void weno5_grad(const double *q, double &un, double &
     const double dp[3][5] = {{1. / 3., -7. / 6., 11. / 6., 0., 0.},
                              {0., -1. / 6., 5. / 6., 1. / 3., 0.},
                              {0., 0., 1. / 3., 5. / 6., -1. / 6.}};
-    const double c[3] = {0.1, 0.6, 0.3};
-    un = weno5_side_grad(p, B, dB, dp, c, gn);
+    const double c[3] = {0.1, 0.6, 0.3}, dd[3] = {1., 6., 3.};
+    un = weno5_side_grad(p, B, dB, dp, c, dd, gn);
     gn[5] = 0.;
   }
   {
@@ -210,8 +226,8 @@ __device__ __forceinline__ void weno5_grad(const double *q, double &un, double &
     const double dp[3][5] = {{-1. / 6., 5. / 6., 1. / 3., 0., 0.},
                              {0., 1. / 3., 5. / 6., -1. / 6., 0.},
                              {0., 0., 11. / 6., -7. / 6., 1. / 3.}};
-    const double c[3] = {0.3, 0.6, 0.1};
-    up = weno5_side_grad(p, B, dB, dp, c, gp + 1);
+    const double c[3] = {0.3, 0.6, 0.1}, dd[3] = {3., 6., 1.};
+    up = weno5_side_grad(p, B, dB, dp, c, dd, gp + 1);
     gp[0] = 0.;
   }
 }

@@ -331,6 +331,30 @@ int build(pda_problem_s &p) {
       bc_factors(p, near_rows[g], a, f);
       for (int k = 0; k < m; ++k) bcf[(g * dim + a) * m + k] = f[k];
     }
+  // structured box: inner cells whose scheme stencil does not wrap
+  p.box = Box();
+  p.h_rem_rows.clear();
+  if (M.structured && dim == 2 &&
+      (p.family == PDA_EULER2D || p.family == PDA_SWE2D || p.family == PDA_BURGERS2D)) {
+    const int wm = (M.stencil - 1) / 2;
+    bool ok = true;
+    for (int a = 0; a < 3; ++a) {
+      p.box.n[a] = M.n[a];
+      if (a >= dim) { p.box.lo[a] = 0; p.box.hi[a] = 1; continue; }
+      const int64_t lo = ((M.periodic_mask >> a) & 1) ? w : wm;
+      p.box.lo[a] = lo;
+      p.box.hi[a] = M.n[a] - lo;
+      if (p.box.hi[a] <= lo) ok = false;
+    }
+    if (ok) {
+      p.box.valid = true;
+      for (int64_t r = 0; r < nr; ++r) {
+        const int64_t i = r % M.n[0], j = r / M.n[0];
+        if (i < p.box.lo[0] || i >= p.box.hi[0] || j < p.box.lo[1] || j >= p.box.hi[1])
+          p.h_rem_rows.push_back(int32_t(r));
+      }
+    }
+  }
   p.h_ghost_row = std::move(ghost_row);
   p.h_near_rows = std::move(near_rows);
   p.h_bcf = std::move(bcf);
@@ -406,6 +430,7 @@ int ensure_device(pda_problem_s &p) {
   PDA_HIP_CHECK(p.d_nblk.upload(p.h_nblk));
   PDA_HIP_CHECK(p.d_slot.upload(p.h_slot));
   PDA_HIP_CHECK(p.d_bcol.upload(p.h_bcol));
+  PDA_HIP_CHECK(p.d_rem_rows.upload(p.h_rem_rows));
   const size_t gn = size_t(2 * p.dim) * size_t(std::max<int64_t>(p.n_near, 1)) * p.w * p.m;
   // unused ghost slots hold numeric_limits<double>::min(),
   // impl/euler_2d_prob_class.hpp:1266-1270

@@ -27,6 +27,20 @@ for s in (S.FirstOrder, S.Weno3):
 
 IDS = [c[0] for c in CASES]
 
+# larger meshes: several 64-wide tiles, partial segments, tall and wide boxes
+BIG_CASES = [
+    ("euler2d-riemann-big-FirstOrder", pda.Euler2d.Riemann, S.FirstOrder, (150, 37), (0, 1, 0, 1), 7, (), 2, None),
+    ("euler2d-riemann-big-Weno5", pda.Euler2d.Riemann, S.Weno5, (150, 37), (0, 1, 0, 1), 7, (), 2, None),
+    ("euler2d-periodic-big-Weno3", pda.Euler2d.PeriodicSmooth, S.Weno3, (70, 131), (-1, 1, -1, 1), 5, True, 0, None),
+    ("swe-big-FirstOrder", pda.Swe2d.SlipWall, S.FirstOrder, (97, 70), (-5, 5, -5, 5), 7, (), 1,
+     {"gravity": 9.8, "coriolis": -3.0}),
+    ("burgers-big-Weno5", pda.AdvectionDiffusion2d.BurgersPeriodic, S.Weno5, (130, 66), (-1, 1, -1, 1), 7,
+     True, 1, {"diffusion": 5e-3}),
+    ("burgers-big-FirstOrder", pda.AdvectionDiffusion2d.BurgersPeriodic, S.FirstOrder, (130, 66), (-1, 1, -1, 1), 3,
+     True, 1, {"diffusion": 5e-3}),
+]
+BIG_IDS = [c[0] for c in BIG_CASES]
+
 ORACLE_PARAMS = {
     1: lambda p: [1.4],
     2: lambda p: [(p or {}).get("gamma", 1.4)],

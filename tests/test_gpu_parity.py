@@ -38,7 +38,7 @@ def _run_gpu(prob, U, torch):
             J.rowptr.cpu().numpy(), J.colidx.cpu().numpy())
 
 
-@pytest.mark.parametrize("case", cases.CASES, ids=cases.IDS)
+@pytest.mark.parametrize("case", cases.CASES + cases.BIG_CASES, ids=cases.IDS + cases.BIG_IDS)
 def test_velocity_and_jacobian_match_oracle(case, oracle, torch):
     print(f"[gpu-parity] {case[0]}", flush=True)
     mesh, prob = cases.make(case)
@@ -51,6 +51,8 @@ def test_velocity_and_jacobian_match_oracle(case, oracle, torch):
         V2o, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, U)
         assert np.array_equal(rp, rpo), "row pointers differ"
         assert np.array_equal(ci, cio), "column indices differ"
+        print(f"[gpu-parity] {case[0]} seed {seed}: V {cases.rel_err(Vg, Vo):.2e} "
+              f"V(J path) {cases.rel_err(V2g, V2o):.2e} J {cases.rel_err(Jg, Jo):.2e}", flush=True)
         assert cases.rel_err(Vg, Vo) <= TOL, cases.rel_err(Vg, Vo)
         assert cases.rel_err(V2g, V2o) <= TOL, cases.rel_err(V2g, V2o)
         assert cases.rel_err(Jg, Jo) <= TOL, cases.rel_err(Jg, Jo)
