@@ -1,87 +1,93 @@
 // pda_box_kernels.hip -- kernels for the cells of a natural-order structured
-// mesh whose whole scheme stencil lies inside the grid (the "box").  They
-// give the same values as the per-row kernels (same face states, same
-// fluxes, same accumulation order) but:
-//   - velocity: every face is reconstructed once per tile and shared by the
-//     two cells it separates through LDS, instead of once per cell side;
-//   - first-order Jacobian: the five m x m blocks of a cell are formed in
-//     registers and written through an LDS stage so that HBM stores are
-//     contiguous across the wavefront.
+// 2D mesh whose whole scheme stencil lies inside the grid (the "box"); the
+// generic row kernels cover the rows outside it.  They compute the same
+// quantities as the per-row kernels (same face states, fluxes and flux
+// Jacobians, same per-cell accumulation order; values agree to rounding)
+// but are laid out for the hardware:
+//   - velocity: one wavefront marches a 63-cell row segment up a strip of
+//     rows, evaluating each face once (x faces shared by lane shuffles, y
+//     faces carried in registers);
+//   - first-order Jacobian: the blocks are assembled in LDS in CSR order and
+//     leave as one contiguous run per row segment (coalesced 16-byte stores);
+//     for 2D Euler two lanes per cell evaluate each face once.
 // Cell ids follow the structured layout gid = j*nx + i (graph row == gid).
-#include "pda_families.hpp"
+#include "pda_rows.hpp"
 
 namespace pda {
 namespace {
 using namespace fam;
 
-// ------------------------------------------------------------ velocity 2D
-constexpr int VTX = 64, VTY = 4; // tile: 64 cells in x, 4 rows in y (256 threads)
+// ------------------------------------------------ velocity 2D, row marching
+// A wavefront owns a segment of up to 63 consecutive box cells of a grid row
+// and marches up a strip of rows.  Lane l evaluates the x face on the left
+// of cell l (lane 63 the face on the right of cell 62) and the y face above
+// cell l; the x face on the right comes from lane l+1 by a lane shuffle and
+// the y face below is the one evaluated for the previous row, carried in
+// registers.  Every face is reconstructed once (plus one row of y faces per
+// strip) and no workgroup barrier is needed.
+constexpr int VSEG = 63;
 
 template <class Ph, int W, int A>
-__device__ __forceinline__ void face_flux_at(const View &P, const double *__restrict__ U,
-                                             int64_t c_right, int64_t stride, double *F) {
-  // face between cells c_right - stride and c_right; stencil c_right+(p-W)*stride, p=0..2W-1
+__device__ __forceinline__ void face_flux_fast(const View &P, const double *__restrict__ U,
+                                               int64_t c_right, int64_t stride, double *F) {
   constexpr int M = Ph::M;
-  double q[2 * W][M];
-#pragma unroll
-  for (int p = 0; p < 2 * W; ++p)
-#pragma unroll
-    for (int k = 0; k < M; ++k) q[p][k] = U[(c_right + int64_t(p - W) * stride) * M + k];
   double qm[M], qp[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     double v[2 * W];
 #pragma unroll
-    for (int p = 0; p < 2 * W; ++p) v[p] = q[p][k];
+    for (int p = 0; p < 2 * W; ++p) v[p] = U[(c_right + int64_t(p - W) * stride) * M + k];
     if constexpr (W == 1) {
       qm[k] = v[0];
       qp[k] = v[1];
     } else if constexpr (W == 2) {
-      phys::weno3(v, qm[k], qp[k]);
+      phys::weno3_1div(v, qm[k], qp[k]);
     } else {
-      phys::weno5(v, qm[k], qp[k]);
+      phys::weno5_1div(v, qm[k], qp[k]);
     }
   }
   Ph::template flux<A>(P, qm, qp, F);
 }
 
 template <class Ph, int W>
-__global__ void __launch_bounds__(256) k_vel_box2d(View P, Box B, const double *__restrict__ U,
-                                                    double *__restrict__ V) {
+__global__ void __launch_bounds__(64) k_vel_march2d(View P, Box B, int rows_per_strip,
+                                                     const double *__restrict__ U,
+                                                     double *__restrict__ V) {
   constexpr int M = Ph::M;
-  __shared__ double fx[VTY][VTX + 1][M];
-  __shared__ double fy[VTY + 1][VTX][M];
+  const int lane = threadIdx.x;
   const int64_t nx = B.n[0];
-  const int64_t i0 = B.lo[0] + int64_t(blockIdx.x) * VTX;
-  const int64_t j0 = B.lo[1] + int64_t(blockIdx.y) * VTY;
-  const int t = threadIdx.x;
-  // x faces: (r, q) is the face on the left of cell (i0+q, j0+r); q = 0..VTX
-  for (int f = t; f < VTY * (VTX + 1); f += 256) {
-    const int r = f / (VTX + 1), q = f % (VTX + 1);
-    const int64_t i = i0 + q, j = j0 + r;
-    if (i > B.hi[0] || j >= B.hi[1]) continue;
-    face_flux_at<Ph, W, 0>(P, U, j * nx + i, 1, fx[r][q]);
+  const int64_t bw = B.hi[0] - B.lo[0];
+  const int64_t nseg = (bw + VSEG - 1) / VSEG;
+  const int64_t seg = int64_t(blockIdx.x) % nseg, strip = int64_t(blockIdx.x) / nseg;
+  const int64_t iseg = B.lo[0] + seg * VSEG;
+  const int ncell = int(bw - seg * VSEG < VSEG ? bw - seg * VSEG : VSEG);
+  const int64_t j0 = B.lo[1] + strip * rows_per_strip;
+  const int64_t j1 = j0 + rows_per_strip < B.hi[1] ? j0 + rows_per_strip : B.hi[1];
+  // idle lanes repeat the last face; iseg + ncell <= B.hi[0], so every
+  // stencil stays inside the grid
+  const int64_t i = iseg + (lane <= ncell ? lane : ncell);
+  const bool own = lane < ncell;
+  const double hx = P.hinv[0], hy = P.hinv[1];
+  double fyb[M];
+  face_flux_fast<Ph, W, 1>(P, U, j0 * nx + i, nx, fyb);
+  for (int64_t j = j0; j < j1; ++j) {
+    const int64_t c = j * nx + i;
+    double fx[M], fy[M], acc[M];
+    face_flux_fast<Ph, W, 0>(P, U, c, 1, fx);
+    face_flux_fast<Ph, W, 1>(P, U, c + nx, nx, fy);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double fr = __shfl_down(fx[k], 1, 64);
+      acc[k] = 0. + hx * (fx[k] - fr);
+      acc[k] += hy * (fyb[k] - fy[k]);
+      fyb[k] = fy[k];
+    }
+    Sources<Ph>::apply(P, c, false, U, int32_t(c), acc, nullptr);
+    if (own) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) V[c * M + k] = acc[k];
+    }
   }
-  // y faces: (r, q) is the face below cell (i0+q, j0+r); r = 0..VTY
-  for (int f = t; f < (VTY + 1) * VTX; f += 256) {
-    const int r = f / VTX, q = f % VTX;
-    const int64_t i = i0 + q, j = j0 + r;
-    if (i >= B.hi[0] || j > B.hi[1]) continue;
-    face_flux_at<Ph, W, 1>(P, U, j * nx + i, nx, fy[r][q]);
-  }
-  __syncthreads();
-  const int q = t % VTX, r = t / VTX;
-  const int64_t i = i0 + q, j = j0 + r;
-  if (i >= B.hi[0] || j >= B.hi[1]) return;
-  const int64_t c = j * nx + i;
-  double acc[M];
-#pragma unroll
-  for (int k = 0; k < M; ++k) acc[k] = 0. + P.hinv[0] * (fx[r][q][k] - fx[r][q + 1][k]);
-#pragma unroll
-  for (int k = 0; k < M; ++k) acc[k] += P.hinv[1] * (fy[r][q][k] - fy[r + 1][q][k]);
-  Sources<Ph>::apply(P, c, false, U, int32_t(c), acc, nullptr);
-#pragma unroll
-  for (int k = 0; k < M; ++k) V[c * M + k] = acc[k];
 }
 
 // ------------------------------------------------- first-order Jacobian 2D
@@ -210,6 +216,189 @@ __global__ void __launch_bounds__(64) k_jac_fo_box2d(View P, Box B, const double
   }
 }
 
+// ------------------------- first-order Euler 2D Jacobian, two lanes per cell
+// A wavefront owns a segment of up to ESEG consecutive box cells of a grid
+// row and marches up a strip of rows.  Lanes 0..31 ("x lanes") evaluate the
+// x faces of the segment (lane l: the face on the left of cell l, lane ncell
+// the face on the right of the last cell), lanes 32..62 ("y lanes") the y
+// face above cell l-32; the y lane carries that face to the next row, where
+// it is the face below.  The 2D Euler flux along y is the flux along x with
+// the two momentum components exchanged, so y lanes swap (rho u, rho v) of
+// both states, run the same x-face code as the x lanes (no divergence) and
+// write entry (a, b) of their blocks to row p(a), column p(b) with p the
+// 1 <-> 2 exchange.  The row's blocks are assembled in LDS in CSR order
+// (per cell: scalar row k, then block [back, left, self, right, front],
+// then column) and copied to the contiguous CSR values of the segment with
+// coalesced 16-byte stores.  Euler has no source terms.
+constexpr int ESEG = 31;
+constexpr int ESTR = 81; // LDS doubles per cell: 80 values + 1 pad against bank conflicts
+
+template <int M>
+__device__ __forceinline__ void load_cell(const double *__restrict__ U, int64_t c, double *q) {
+#pragma unroll
+  for (int k = 0; k < M; ++k) q[k] = U[c * M + k];
+}
+
+__device__ __forceinline__ void swap_mom(bool sw, double *q) {
+  const double a = q[1], b = q[2];
+  q[1] = sw ? b : a;
+  q[2] = sw ? a : b;
+}
+
+template <class S>
+__device__ __forceinline__ S pick(bool c, const S &a, const S &b) {
+  S r;
+  const double *pa = reinterpret_cast<const double *>(&a);
+  const double *pb = reinterpret_cast<const double *>(&b);
+  double *pr = reinterpret_cast<double *>(&r);
+#pragma unroll
+  for (int q = 0; q < int(sizeof(S) / sizeof(double)); ++q) pr[q] = c ? pa[q] : pb[q];
+  return r;
+}
+
+template <class S>
+__device__ __forceinline__ S shfl_next(const S &a) {
+  S r;
+  const double *pa = reinterpret_cast<const double *>(&a);
+  double *pr = reinterpret_cast<double *>(&r);
+#pragma unroll
+  for (int q = 0; q < int(sizeof(S) / sizeof(double)); ++q) pr[q] = __shfl_down(pa[q], 1, 64);
+  return r;
+}
+
+__device__ __forceinline__ void jac_fo_euler2d_seg(const View &P, const Box &B, int rows_per_strip,
+                                                  int64_t bid, const double *__restrict__ U,
+                                                  double *__restrict__ J) {
+  using Ph = PhEuler2;
+  using Side = Ph::Side;
+  constexpr int M = 4, PER = 80;
+  __shared__ double stage[ESEG * ESTR];
+  const int lane = threadIdx.x;
+  const bool yl = lane >= 32;
+  const int l = lane & 31;
+  const int64_t nx = B.n[0];
+  const int64_t bw = B.hi[0] - B.lo[0];
+  const int64_t nseg = (bw + ESEG - 1) / ESEG;
+  const int64_t seg = bid % nseg, strip = bid / nseg;
+  const int64_t iseg = B.lo[0] + seg * ESEG;
+  const int ncell = int(bw - seg * ESEG < ESEG ? bw - seg * ESEG : ESEG);
+  const int64_t j0 = B.lo[1] + strip * rows_per_strip;
+  const int64_t j1 = j0 + rows_per_strip < B.hi[1] ? j0 + rows_per_strip : B.hi[1];
+  // x lane l: face between cells l-1 and l (valid l <= ncell); y lane l: cell l (valid l < ncell)
+  const int ci = yl ? (l < ncell ? l : ncell - 1) : (l <= ncell ? l : ncell);
+  const int64_t da = yl ? 0 : -1;       // first state: cell on the left (x) or the cell itself (y)
+  const int64_t db = yl ? nx : 1;       // second state relative to the first
+  const double h = yl ? P.hinv[1] : P.hinv[0];
+  // block 1: x lane -> left block of cell l (slot 1, +h JL); y lane -> front block (slot 4, -h JR)
+  // block 2: x lane -> right block of cell l-1 (slot 3, -h JR); y lane -> back block (slot 0, +h JL below)
+  const bool v1 = yl ? l < ncell : l < ncell;
+  const bool v2 = yl ? l < ncell : (l >= 1 && l <= ncell);
+  const int c1 = l, c2 = yl ? l : l - 1;
+  const int s1 = yl ? 4 : 1, s2 = yl ? 0 : 3;
+  const double g1 = yl ? -h : h, g2 = yl ? h : -h;
+  // LDS offset of entry (a, b) of a block, with the y lanes' exchange of 1 and 2
+  auto pr = [&](int a) { return yl ? (a == 1 ? 2 : a == 2 ? 1 : a) : a; };
+  Side bl0, bl1; // y lanes: the face below the current row (left, right state)
+  {
+    double qa[M], qb[M];
+    const int64_t c = (j0 - 1) * nx + iseg + ci;
+    load_cell<M>(U, c, qa);
+    load_cell<M>(U, c + nx, qb);
+    swap_mom(yl, qa);
+    swap_mom(yl, qb);
+    Ph::sides<0>(P, qa, qb, bl0, bl1);
+  }
+  double qa[M], qb[M];
+  {
+    const int64_t c = j0 * nx + iseg + ci + da;
+    load_cell<M>(U, c, qa);
+    load_cell<M>(U, c + db, qb);
+  }
+  for (int64_t j = j0; j < j1; ++j) {
+    swap_mom(yl, qa);
+    swap_mom(yl, qb);
+    Side S0, S1;
+    Ph::sides<0>(P, qa, qb, S0, S1);
+    const Side nS0 = shfl_next(S0); // x lanes: left-state side of the next x face
+    {
+      const Side X1 = pick(yl, S1, S0);
+      double *d1 = stage + c1 * ESTR + s1 * M;
+      if (v1) {
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+          for (int b = 0; b < M; ++b) d1[pr(a) * 5 * M + pr(b)] = g1 * Ph::entry<0>(P, X1, a, b);
+      }
+      const Side X2 = pick(yl, bl0, S1);
+      double *d2 = stage + c2 * ESTR + s2 * M;
+      if (v2) {
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+          for (int b = 0; b < M; ++b) d2[pr(a) * 5 * M + pr(b)] = g2 * Ph::entry<0>(P, X2, a, b);
+      }
+    }
+    // diagonal block: x part (JR of the left face - JL of the right face) * hx
+    // written by the x lanes, then the y part (JR below - JL above) * hy
+    // added by the y lanes
+    {
+      const Side Y1 = pick(yl, bl1, S1), Y2 = pick(yl, S0, nS0);
+      double part[M * M];
+#pragma unroll
+      for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int b = 0; b < M; ++b)
+          part[a * M + b] = (Ph::entry<0>(P, Y1, a, b) - Ph::entry<0>(P, Y2, a, b)) * h;
+      double *ds = stage + l * ESTR + 2 * M;
+      if (!yl && l < ncell) {
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+          for (int b = 0; b < M; ++b) ds[a * 5 * M + b] = part[a * M + b];
+      }
+      wave_sync();
+      if (yl && l < ncell) {
+#pragma unroll
+        for (int a = 0; a < M; ++a)
+#pragma unroll
+          for (int b = 0; b < M; ++b) ds[pr(a) * 5 * M + pr(b)] += part[a * M + b];
+      }
+    }
+    bl0 = S0;
+    bl1 = S1;
+    wave_sync();
+    if (j + 1 < j1) {
+      const int64_t c = (j + 1) * nx + iseg + ci + da;
+      load_cell<M>(U, c, qa);
+      load_cell<M>(U, c + db, qb);
+    }
+    // copy the row's blocks out: 16-byte chunks, 40 per cell
+    double2 *dst = reinterpret_cast<double2 *>(J + P.rowbase[j * nx + iseg]);
+    for (int q = lane; q < ncell * (PER / 2); q += 64) {
+      const int cc = q / (PER / 2), o = q - cc * (PER / 2);
+      const double *src = stage + cc * ESTR + 2 * o;
+      dst[q] = make_double2(src[0], src[1]);
+    }
+    wave_sync();
+  }
+}
+
+// One launch for the whole first-order Euler 2D Jacobian: the first
+// nframe workgroups run the generic row code over the rows outside the box
+// (boundary ring, ghosts and BC factors), the rest the box segments.  Lower
+// workgroup ids are dispatched first, so the latency-bound ring rows start
+// ahead of the box and finish in its shadow; both paths need about the same
+// registers, so the fusion costs the box no occupancy.
+__global__ void __launch_bounds__(64) k_jac_fo_euler2d(View P, Box B, int rows_per_strip,
+                                                        int nframe, const double *__restrict__ U,
+                                                        double *__restrict__ J) {
+  if (int(blockIdx.x) < nframe) {
+    rows::jacobian_row<PhEuler2, 1, false>(P, int64_t(blockIdx.x) * 64 + threadIdx.x, U, nullptr, J);
+    return;
+  }
+  jac_fo_euler2d_seg(P, B, rows_per_strip, int64_t(blockIdx.x) - nframe, U, J);
+}
+
 }  // namespace
 
 bool has_box_kernels(const pda_problem_s &p) {
@@ -220,11 +409,15 @@ bool has_box_kernels(const pda_problem_s &p) {
 template <class Ph>
 static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
   const Box &B = p.box;
-  const dim3 grid(unsigned((B.hi[0] - B.lo[0] + VTX - 1) / VTX),
-                  unsigned((B.hi[1] - B.lo[1] + VTY - 1) / VTY));
-  if (p.w == 1) hipLaunchKernelGGL((k_vel_box2d<Ph, 1>), grid, dim3(256), 0, s, p.view, B, U, V);
-  else if (p.w == 2) hipLaunchKernelGGL((k_vel_box2d<Ph, 2>), grid, dim3(256), 0, s, p.view, B, U, V);
-  else hipLaunchKernelGGL((k_vel_box2d<Ph, 3>), grid, dim3(256), 0, s, p.view, B, U, V);
+  // strips of rows_per_strip rows: ~8k one-wave groups for the 1024^2 grid
+  const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
+  const int64_t bh = B.hi[1] - B.lo[1];
+  int64_t rps = (bh * nseg) / 8192;
+  rps = rps < 1 ? 1 : (rps > 64 ? 64 : rps);
+  const dim3 grid(unsigned(nseg * ((bh + rps - 1) / rps)));
+  if (p.w == 1) hipLaunchKernelGGL((k_vel_march2d<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  else if (p.w == 2) hipLaunchKernelGGL((k_vel_march2d<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  else hipLaunchKernelGGL((k_vel_march2d<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
@@ -240,8 +433,26 @@ int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipS
 }
 
 template <class Ph>
-static int jac_box(const pda_problem_s &p, const double *U, double *J, double *V, hipStream_t s) {
+static int jac_box(const pda_problem_s &p, const double *U, double *J, double *V, hipStream_t s,
+                   bool *rest_done) {
   const Box &B = p.box;
+  if constexpr (std::is_same<Ph, PhEuler2>::value) {
+    if (V == nullptr) {
+      // strips of rows_per_strip rows: ~16k one-wave groups for the 1024^2 grid
+      const int64_t nseg = (B.hi[0] - B.lo[0] + ESEG - 1) / ESEG;
+      const int64_t bh = B.hi[1] - B.lo[1];
+      int64_t rps = (bh * nseg) / 16384;
+      rps = rps < 1 ? 1 : (rps > 64 ? 64 : rps);
+      const int64_t nstrip = (bh + rps - 1) / rps;
+      const View v = generic_view(p, true);
+      const int64_t nframe = (v.nvisit + 63) / 64;
+      hipLaunchKernelGGL(k_jac_fo_euler2d, dim3(unsigned(nframe + nseg * nstrip)), dim3(64), 0, s, v,
+                         B, int(rps), int(nframe), U, J);
+      PDA_HIP_CHECK(hipGetLastError());
+      *rest_done = true;
+      return PDA_OK;
+    }
+  }
   const int64_t nseg = (B.hi[0] - B.lo[0] + 63) / 64;
   const int64_t nblocks = nseg * (B.hi[1] - B.lo[1]);
   hipLaunchKernelGGL((k_jac_fo_box2d<Ph>), dim3(unsigned(nblocks)), dim3(64), 0, s, p.view, B, U, V, J);
@@ -250,13 +461,14 @@ static int jac_box(const pda_problem_s &p, const double *U, double *J, double *V
 }
 
 int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, double *V,
-                        hipStream_t s) {
+                        hipStream_t s, bool *rest_done) {
+  *rest_done = false;
   if (p.w != 1) return PDA_E_UNSUPPORTED;
   if (!p.box.valid || p.box.cells() == 0) return PDA_OK;
   switch (p.family) {
-    case PDA_EULER2D: return jac_box<PhEuler2>(p, U, J, V, s);
-    case PDA_SWE2D: return jac_box<PhSwe>(p, U, J, V, s);
-    case PDA_BURGERS2D: return jac_box<PhBurgers>(p, U, J, V, s);
+    case PDA_EULER2D: return jac_box<PhEuler2>(p, U, J, V, s, rest_done);
+    case PDA_SWE2D: return jac_box<PhSwe>(p, U, J, V, s, rest_done);
+    case PDA_BURGERS2D: return jac_box<PhBurgers>(p, U, J, V, s, rest_done);
   }
   return PDA_E_UNSUPPORTED;
 }

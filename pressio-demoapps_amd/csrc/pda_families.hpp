@@ -11,40 +11,53 @@ namespace fam {
 // ------------------------------------------------------------------ physics
 // Each family: M dofs per cell, DIM axes, face flux and its Jacobians for
 // axis A, and the per-cell source terms applied after the axis sweep.
-struct PhEuler1 {
-  static constexpr int M = 3, DIM = 1;
+// sides()/entry() give the face Jacobian blocks in the family's compact form
+// (Side): entry(S, i, j) is element (i, j) of that side's block, with i, j
+// compile-time after unrolling.
+
+// a side's block held as its M x M values
+template <int M> struct MatSide {
+  double J[M * M];
+};
+
+template <int D> struct PhEuler {
+  static constexpr int M = D + 2, DIM = D;
+  using Side = phys::EulerSide<D>;
   template <int A>
   __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::euler_flux<1, A>(l, r, P.p[0], F);
+    phys::euler_flux<D, A>(l, r, P.p[0], F);
   }
   template <int A>
   __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::euler_fjac<1, A>(l, r, P.p[0], JL, JR);
+    phys::euler_fjac<D, A>(l, r, P.p[0], JL, JR);
+  }
+  template <int A>
+  __device__ static void sides(const View &P, const double *l, const double *r, Side &L, Side &R) {
+    phys::euler_sides<D, A>(l, r, P.p[0], L, R);
+  }
+  template <int A>
+  __device__ static double entry(const View &P, const Side &S, int i, int j) {
+    return phys::euler_side_entry<D, A>(S, P.p[0], i, j);
   }
 };
-struct PhEuler2 {
-  static constexpr int M = 4, DIM = 2;
+using PhEuler1 = PhEuler<1>;
+using PhEuler2 = PhEuler<2>;
+using PhEuler3 = PhEuler<3>;
+
+// families whose face blocks are small enough to hold as values
+template <class Derived, int M_> struct MatSided {
+  using Side = MatSide<M_>;
   template <int A>
-  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::euler_flux<2, A>(l, r, P.p[0], F);
+  __device__ static void sides(const View &P, const double *l, const double *r, Side &L, Side &R) {
+    Derived::template fjac<A>(P, l, r, L.J, R.J);
   }
   template <int A>
-  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::euler_fjac<2, A>(l, r, P.p[0], JL, JR);
-  }
-};
-struct PhEuler3 {
-  static constexpr int M = 5, DIM = 3;
-  template <int A>
-  __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
-    phys::euler_flux<3, A>(l, r, P.p[0], F);
-  }
-  template <int A>
-  __device__ static void fjac(const View &P, const double *l, const double *r, double *JL, double *JR) {
-    phys::euler_fjac<3, A>(l, r, P.p[0], JL, JR);
+  __device__ static double entry(const View &, const Side &S, int i, int j) {
+    return S.J[i * M_ + j];
   }
 };
-struct PhSwe {
+
+struct PhSwe : MatSided<PhSwe, 3> {
   static constexpr int M = 3, DIM = 2;
   template <int A>
   __device__ static void flux(const View &P, const double *l, const double *r, double *F) {
@@ -55,7 +68,7 @@ struct PhSwe {
     phys::swe_fjac<A>(l, r, P.p[0], JL, JR);
   }
 };
-struct PhBurgers {
+struct PhBurgers : MatSided<PhBurgers, 2> {
   static constexpr int M = 2, DIM = 2;
   template <int A>
   __device__ static void flux(const View &, const double *l, const double *r, double *F) {

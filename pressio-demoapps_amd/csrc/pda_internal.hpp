@@ -167,12 +167,11 @@ struct pda_problem_s {
   pda::View view;
   pda::DevBuf d_graph, d_ghost_row, d_near_rows, d_ghost, d_bcf;
   pda::DevBuf d_rowptr, d_colidx, d_rowbase, d_nblk, d_slot, d_bcol;
-  pda::DevBuf d_scratch_v;           // V buffer for jacobian() without V
+  pda::DevBuf d_scratch_v;           // Jacobian values for applyJacobian()
 };
 
 namespace pda {
 int ensure_device(pda_problem_s &p);
-int launch_ghosts(const pda_problem_s &p, const double *U, hipStream_t s);
 int launch_velocity(const pda_problem_s &p, const double *U, double t, double *V, hipStream_t s);
 int launch_jacobian(const pda_problem_s &p, const double *U, double t, double *J,
                     double *V, hipStream_t s);
@@ -181,7 +180,10 @@ int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64
 // structured-box kernels; return PDA_E_UNSUPPORTED when the problem has no
 // box kernel (the caller then covers all rows with the generic kernels)
 int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s);
+// rest_done: the launch also covered the rows outside the box
 int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, double *V,
-                        hipStream_t s);
+                        hipStream_t s, bool *rest_done);
+// view of the generic kernels: every row, or only the rows outside the box
+View generic_view(const pda_problem_s &p, bool box_done);
 bool has_box_kernels(const pda_problem_s &p);
 }  // namespace pda

@@ -1,290 +1,24 @@
-// pda_kernels.hip -- gfx950 kernels of the finite-volume hot path:
-// ghost fill, velocity (residual), velocity+Jacobian assembly into the fixed
-// CSR pattern, and the Jacobian action.
-//
-// One thread owns one graph row (one sample-mesh cell) and produces all of
-// its V entries and all of its Jacobian block-row, so every output value is
-// written exactly once with plain stores (no atomics, no zero-fill pass);
-// the per-entry accumulation order follows the reference:
-//   - V: x, then y, then z axis, then sources
-//     (impl/mixin_directional_flux_balance.hpp:68-84);
-//   - interior Jacobian: chain rule flux-Jacobian x reconstruction gradient,
-//     left face then right face per axis
-//     (impl/mixin_directional_flux_balance_jacobian.hpp:142-284);
-//   - near-boundary rows: first-order blocks with BC factors folded into the
-//     diagonal block (impl/mixin_directional_flux_balance_jacobian.hpp:311-371).
-#include "pda_families.hpp"
+// pda_kernels.hip -- generic kernels of the finite-volume hot path (any
+// mesh: full, sample/stencil, 1D/2D/3D): velocity (residual), Jacobian
+// assembly into the fixed CSR pattern, and the Jacobian action, plus the
+// launch logic that pairs them with the structured box kernels.  The per-row
+// device code is in pda_rows.hpp.
+#include "pda_rows.hpp"
 
 namespace pda {
 namespace {
 using namespace fam;
 
-// ---------------------------------------------------------------- ghosts
-// Ghost layers of every near-boundary row.  Layer 0 copies the cell itself;
-// for layer L>0 the source cell follows the family's rule:
-//   GHOST_OPPOSITE: the opposite layer-(L-1) neighbour
-//     (impl/euler_1d_ghost_filler.hpp:104-143, impl/euler_3d_ghost_filler_sedov.hpp:184-236,
-//      impl/advection_diffusion_reaction_2d_ghost_filler_problemA.hpp:106-133);
-//   GHOST_MIRROR: the mirror image about the boundary
-//     (impl/euler_2d_ghost_filler_neumann.hpp:163-279,
-//      impl/swe_2d_ghost_filler_inviscid_wall.hpp:160-270);
-// each value is multiplied by the side's sign pattern (reflective walls,
-// Dirichlet negation).  A slot is written only when its neighbour is -1,
-// the same condition under which the stencil reads it.
-template <int DIM, int M>
-__global__ void k_ghosts(View P, int W, const double *__restrict__ U) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= P.n_near) return;
-  const int64_t r = P.near_rows[g];
-  const int32_t *gr = P.graph + r * P.ncols;
-  const int32_t self = gr[0];
-  for (int a = 0; a < DIM; ++a)
-    for (int side = 0; side < 2; ++side) {
-      double *dst = P.ghost + ((int64_t(a) * 2 + side) * P.n_near + g) * (W * M);
-      const double *sg = P.gsign[a][side];
-      for (int L = 0; L < W; ++L) {
-        const int32_t nL = gr[colof<DIM>(a, side, L)];
-        if (nL != -1) continue;
-        int32_t src = self;
-        if (L > 0) {
-          if (P.ghost_rule == GHOST_OPPOSITE) {
-            src = gr[colof<DIM>(a, 1 - side, L - 1)];
-          } else {
-            const int32_t n0 = gr[colof<DIM>(a, side, 0)];
-            const int32_t o0 = gr[colof<DIM>(a, 1 - side, 0)];
-            if (L == 1) {
-              src = n0 == -1 ? o0 : n0;
-            } else {
-              const int32_t n1 = gr[colof<DIM>(a, side, 1)];
-              const int32_t o1 = gr[colof<DIM>(a, 1 - side, 1)];
-              if (n1 != -1 && n0 != -1) src = n1;
-              if (n1 == -1 && n0 != -1) src = self;
-              if (n1 == -1 && n0 == -1) src = o1;
-            }
-          }
-        }
-        if (src < 0) src = self; // degenerate meshes narrower than the stencil
-        for (int k = 0; k < M; ++k) dst[L * M + k] = sg[k] * U[int64_t(src) * M + k];
-      }
-    }
-}
-
-// ---------------------------------------------------------------- velocity
-template <class Ph, int W, int A>
-__device__ __forceinline__ void velocity_axis(const View &P, const int32_t *gr, int64_t g,
-                                              const double *__restrict__ U, double *acc) {
-  constexpr int M = Ph::M;
-  double st[2 * W + 1][M];
-  gather<Ph, W, A>(P, gr, g, W, U, st);
-  Faces<M, W> f;
-  reconstruct<M, W>(st, f);
-  double FL[M], FR[M];
-  Ph::template flux<A>(P, f.lm, f.lp, FL);
-  Ph::template flux<A>(P, f.rm, f.rp, FR);
-  const double h = P.hinv[A];
-#pragma unroll
-  for (int k = 0; k < M; ++k) acc[k] += h * (FL[k] - FR[k]);
-}
-
 template <class Ph, int W>
-__global__ void __launch_bounds__(256) k_velocity(View P, const double *__restrict__ U,
+__global__ void __launch_bounds__(64) k_velocity(View P, const double *__restrict__ U,
                                                    double *__restrict__ V) {
-  constexpr int M = Ph::M;
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= P.nvisit) return;
-  const int64_t r = P.rows ? P.rows[t] : t;
-  const int32_t *gr = P.graph + r * P.ncols;
-  const int64_t g = P.ghost_row[r];
-  double acc[M];
-#pragma unroll
-  for (int k = 0; k < M; ++k) acc[k] = 0.;
-  velocity_axis<Ph, W, 0>(P, gr, g, U, acc);
-  if constexpr (Ph::DIM > 1) velocity_axis<Ph, W, 1>(P, gr, g, U, acc);
-  if constexpr (Ph::DIM > 2) velocity_axis<Ph, W, 2>(P, gr, g, U, acc);
-  Sources<Ph>::apply(P, r, g >= 0, U, gr[0], acc, nullptr);
-#pragma unroll
-  for (int k = 0; k < M; ++k) V[r * M + k] = acc[k];
+  rows::velocity_row<Ph, W>(P, int64_t(blockIdx.x) * blockDim.x + threadIdx.x, U, V);
 }
 
-// ---------------------------------------------------------------- jacobian
-// interior row, one axis
-template <class Ph, int W, int A>
-__device__ __forceinline__ void jac_axis_inner(const View &P, const int32_t *gr,
-                                               const uint8_t *sl, const double *__restrict__ U,
-                                               double *acc, double (&self)[Ph::M][Ph::M],
-                                               double *__restrict__ J, int64_t base, int rs) {
-  constexpr int M = Ph::M;
-  double st[2 * W + 1][M];
-  gather<Ph, W, A>(P, gr, -1, W, U, st);
-  Faces<M, W> f;
-  double FL[M], FR[M];
-  double JLn[M * M], JLp[M * M], JRn[M * M], JRp[M * M];
-  const double h = P.hinv[A];
-  if constexpr (W == 1) {
-    reconstruct<M, W>(st, f);
-    Ph::template flux<A>(P, f.lm, f.lp, FL);
-    Ph::template flux<A>(P, f.rm, f.rp, FR);
-    Ph::template fjac<A>(P, f.lm, f.lp, JLn, JLp);
-    Ph::template fjac<A>(P, f.rm, f.rp, JRn, JRp);
-#pragma unroll
-    for (int k = 0; k < M; ++k) acc[k] += h * (FL[k] - FR[k]);
-    double bl[M][M], br[M][M];
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-#pragma unroll
-      for (int j = 0; j < M; ++j) {
-        bl[k][j] = 0. + JLn[k * M + j] * h + Sources<Ph>::offdiag(P, false, A, k, j);
-        self[k][j] += (JLp[k * M + j] - JRn[k * M + j]) * h;
-        br[k][j] = 0. + (-JRp[k * M + j]) * h + Sources<Ph>::offdiag(P, false, A, k, j);
-      }
-    put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 0, 0)], bl);
-    put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 1, 0)], br);
-  } else {
-#pragma clang fp contract(off)
-    FaceGrads<M, W> G;
-    reconstruct_grad<M, W>(st, f, G);
-    Ph::template flux<A>(P, f.lm, f.lp, FL);
-    Ph::template flux<A>(P, f.rm, f.rp, FR);
-    Ph::template fjac<A>(P, f.lm, f.lp, JLn, JLp);
-    Ph::template fjac<A>(P, f.rm, f.rp, JRn, JRp);
-#pragma unroll
-    for (int k = 0; k < M; ++k) acc[k] += h * (FL[k] - FR[k]);
-    // stencil point p (0..2W) collects the left face's gradient index p and
-    // the right face's index p-1
-#pragma unroll
-    for (int p = 0; p <= 2 * W; ++p) {
-      double b[M][M];
-#pragma unroll
-      for (int k = 0; k < M; ++k)
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-          double v = (p == W) ? self[k][j] : 0.;
-          if (p < 2 * W) {
-            v += JLn[k * M + j] * G.glm[j][p] * h;
-            v += JLp[k * M + j] * G.glp[j][p] * h;
-          }
-          if (p > 0) {
-            v -= JRn[k * M + j] * G.grm[j][p - 1] * h;
-            v -= JRp[k * M + j] * G.grp[j][p - 1] * h;
-          }
-          if (p == W - 1 || p == W + 1) v += Sources<Ph>::offdiag(P, false, A, k, j);
-          b[k][j] = v;
-        }
-      if (p == W) {
-#pragma unroll
-        for (int k = 0; k < M; ++k)
-#pragma unroll
-          for (int j = 0; j < M; ++j) self[k][j] = b[k][j];
-      } else {
-        const int side = p > W ? 1 : 0;
-        const int L = p > W ? p - W - 1 : W - 1 - p;
-        put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, side, L)], b);
-      }
-    }
-  }
-}
-
-// near-boundary row, one axis: velocity from the scheme's stencil (ghosts
-// for missing cells), Jacobian first order with the BC factors
-template <class Ph, int W, int A>
-__device__ __forceinline__ void jac_axis_near(const View &P, const int32_t *gr, int64_t g,
-                                              const uint8_t *sl, const double *__restrict__ U,
-                                              double *acc, double (&self)[Ph::M][Ph::M],
-                                              double *__restrict__ J, int64_t base, int rs) {
-  constexpr int M = Ph::M;
-  const double h = P.hinv[A];
-  double FL[M], FR[M];
-  double st1[3][M];
-  gather<Ph, 1, A>(P, gr, g, W, U, st1);
-  Faces<M, 1> f1;
-  reconstruct<M, 1>(st1, f1);
-  if constexpr (W == 1) {
-    Ph::template flux<A>(P, f1.lm, f1.lp, FL);
-    Ph::template flux<A>(P, f1.rm, f1.rp, FR);
-  } else {
-    double st[2 * W + 1][M];
-    gather<Ph, W, A>(P, gr, g, W, U, st);
-    Faces<M, W> f;
-    reconstruct<M, W>(st, f);
-    Ph::template flux<A>(P, f.lm, f.lp, FL);
-    Ph::template flux<A>(P, f.rm, f.rp, FR);
-  }
-#pragma unroll
-  for (int k = 0; k < M; ++k) acc[k] += h * (FL[k] - FR[k]);
-  double JLn[M * M], JLp[M * M], JRn[M * M], JRp[M * M];
-  Ph::template fjac<A>(P, f1.lm, f1.lp, JLn, JLp);
-  Ph::template fjac<A>(P, f1.rm, f1.rp, JRn, JRp);
-  const int32_t l0 = gr[colof<Ph::DIM>(A, 0, 0)], r0 = gr[colof<Ph::DIM>(A, 1, 0)];
-  const double *fac = P.bcf + (g * Ph::DIM + A) * M;
-#pragma unroll
-  for (int k = 0; k < M; ++k)
-#pragma unroll
-    for (int j = 0; j < M; ++j) self[k][j] += (JLp[k * M + j] - JRn[k * M + j]) * h;
-  if (l0 != -1) {
-    double b[M][M];
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-#pragma unroll
-      for (int j = 0; j < M; ++j) b[k][j] = 0. + JLn[k * M + j] * h;
-    put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 0, 0)], b);
-  }
-  if (r0 != -1) {
-    double b[M][M];
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-#pragma unroll
-      for (int j = 0; j < M; ++j) b[k][j] = 0. + (-JRp[k * M + j]) * h;
-    put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 1, 0)], b);
-  }
-  if (l0 == -1) {
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-#pragma unroll
-      for (int j = 0; j < M; ++j) self[k][j] += (fac[j] * JLn[k * M + j]) * h;
-  }
-  if (r0 == -1) {
-#pragma unroll
-    for (int k = 0; k < M; ++k)
-#pragma unroll
-      for (int j = 0; j < M; ++j) self[k][j] += (fac[j] * -JRp[k * M + j]) * h;
-  }
-}
-
-template <class Ph, int W>
-__global__ void __launch_bounds__(256) k_jacobian(View P, const double *__restrict__ U,
+template <class Ph, int W, bool WV>
+__global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restrict__ U,
                                                    double *__restrict__ V, double *__restrict__ J) {
-  constexpr int M = Ph::M;
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= P.nvisit) return;
-  const int64_t r = P.rows ? P.rows[t] : t;
-  const int32_t *gr = P.graph + r * P.ncols;
-  const int64_t g = P.ghost_row[r];
-  const uint8_t *sl = P.slot + r * P.npos;
-  const int64_t base = P.rowbase[r];
-  const int rs = P.nblk[r] * M;
-  double acc[M];
-  double self[M][M];
-#pragma unroll
-  for (int k = 0; k < M; ++k) {
-    acc[k] = 0.;
-#pragma unroll
-    for (int j = 0; j < M; ++j) self[k][j] = 0.;
-  }
-  if (g < 0) {
-    jac_axis_inner<Ph, W, 0>(P, gr, sl, U, acc, self, J, base, rs);
-    if constexpr (Ph::DIM > 1) jac_axis_inner<Ph, W, 1>(P, gr, sl, U, acc, self, J, base, rs);
-    if constexpr (Ph::DIM > 2) jac_axis_inner<Ph, W, 2>(P, gr, sl, U, acc, self, J, base, rs);
-  } else {
-    jac_axis_near<Ph, W, 0>(P, gr, g, sl, U, acc, self, J, base, rs);
-    if constexpr (Ph::DIM > 1) jac_axis_near<Ph, W, 1>(P, gr, g, sl, U, acc, self, J, base, rs);
-    if constexpr (Ph::DIM > 2) jac_axis_near<Ph, W, 2>(P, gr, g, sl, U, acc, self, J, base, rs);
-  }
-  Sources<Ph>::apply(P, r, g >= 0, U, gr[0], acc, &self[0][0]);
-  put_block<M>(P, J, base, rs, sl[0], self);
-  if (V) {
-#pragma unroll
-    for (int k = 0; k < M; ++k) V[r * M + k] = acc[k];
-  }
+  rows::jacobian_row<Ph, W, WV>(P, int64_t(blockIdx.x) * blockDim.x + threadIdx.x, U, V, J);
 }
 
 // ------------------------------------------------- Jacobian action (SpMM)
@@ -308,25 +42,9 @@ __global__ void k_apply(const int64_t *__restrict__ rowptr, const int32_t *__res
 
 }  // namespace
 
-int launch_ghosts(const pda_problem_s &p, const double *U, hipStream_t s) {
-  if (p.view.ghost_rule == GHOST_NONE || p.n_near == 0) return PDA_OK;
-  const int bs = 128;
-  const unsigned nb = unsigned((p.n_near + bs - 1) / bs);
-  switch (p.family) {
-    case PDA_EULER1D: hipLaunchKernelGGL((k_ghosts<1, 3>), dim3(nb), dim3(bs), 0, s, p.view, p.w, U); break;
-    case PDA_EULER2D: hipLaunchKernelGGL((k_ghosts<2, 4>), dim3(nb), dim3(bs), 0, s, p.view, p.w, U); break;
-    case PDA_EULER3D: hipLaunchKernelGGL((k_ghosts<3, 5>), dim3(nb), dim3(bs), 0, s, p.view, p.w, U); break;
-    case PDA_SWE2D: hipLaunchKernelGGL((k_ghosts<2, 3>), dim3(nb), dim3(bs), 0, s, p.view, p.w, U); break;
-    case PDA_BURGERS2D: hipLaunchKernelGGL((k_ghosts<2, 2>), dim3(nb), dim3(bs), 0, s, p.view, p.w, U); break;
-    default: set_error("ghost fill: unsupported family"); return PDA_E_UNSUPPORTED;
-  }
-  PDA_HIP_CHECK(hipGetLastError());
-  return PDA_OK;
-}
-
 template <class Ph>
 static int vel_dispatch(const View &v, int w, const double *U, double *V, hipStream_t s) {
-  const int bs = 256;
+  const int bs = 64; // one-wave groups fit next to the structured kernels
   if (v.nvisit == 0) return PDA_OK;
   const unsigned nb = unsigned((v.nvisit + bs - 1) / bs);
   if (w == 1) hipLaunchKernelGGL((k_velocity<Ph, 1>), dim3(nb), dim3(bs), 0, s, v, U, V);
@@ -339,12 +57,18 @@ static int vel_dispatch(const View &v, int w, const double *U, double *V, hipStr
 template <class Ph>
 static int jac_dispatch(const View &v, int w, const double *U, double *V, double *J,
                         hipStream_t s) {
-  const int bs = 256;
+  const int bs = 64; // one-wave groups fit next to the structured kernels
   if (v.nvisit == 0) return PDA_OK;
   const unsigned nb = unsigned((v.nvisit + bs - 1) / bs);
-  if (w == 1) hipLaunchKernelGGL((k_jacobian<Ph, 1>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
-  else if (w == 2) hipLaunchKernelGGL((k_jacobian<Ph, 2>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
-  else hipLaunchKernelGGL((k_jacobian<Ph, 3>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+  if (V) {
+    if (w == 1) hipLaunchKernelGGL((k_jacobian<Ph, 1, true>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+    else if (w == 2) hipLaunchKernelGGL((k_jacobian<Ph, 2, true>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+    else hipLaunchKernelGGL((k_jacobian<Ph, 3, true>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+  } else {
+    if (w == 1) hipLaunchKernelGGL((k_jacobian<Ph, 1, false>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+    else if (w == 2) hipLaunchKernelGGL((k_jacobian<Ph, 2, false>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+    else hipLaunchKernelGGL((k_jacobian<Ph, 3, false>), dim3(nb), dim3(bs), 0, s, v, U, V, J);
+  }
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
@@ -357,7 +81,7 @@ __global__ void k_fill(double *x, int64_t n, double v) {
 }
 
 // the rows the generic kernels cover: all of them, or those outside the box
-static View generic_view(const pda_problem_s &p, bool box_done) {
+View generic_view(const pda_problem_s &p, bool box_done) {
   View v = p.view;
   if (box_done) {
     v.rows = p.d_rem_rows.as<int32_t>();
@@ -369,17 +93,8 @@ static View generic_view(const pda_problem_s &p, bool box_done) {
   return v;
 }
 
-int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, hipStream_t s) {
-  if (p.view.nrows == 0) return PDA_OK;
-  int rc = launch_ghosts(p, U, s);
-  if (rc) return rc;
-  bool box_done = false;
-  if (p.use_box && p.box.valid && has_box_kernels(p)) {
-    rc = launch_velocity_box(p, U, V, s);
-    if (rc == PDA_OK) box_done = true;
-    else if (rc != PDA_E_UNSUPPORTED) return rc;
-  }
-  const View v = generic_view(p, box_done);
+static int vel_generic(const View &v, const pda_problem_s &p, const double *U, double *V,
+                       hipStream_t s) {
   switch (p.family) {
     case PDA_EULER1D: return vel_dispatch<PhEuler1>(v, p.w, U, V, s);
     case PDA_EULER2D: return vel_dispatch<PhEuler2>(v, p.w, U, V, s);
@@ -390,22 +105,8 @@ int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, 
   }
 }
 
-int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, double *V,
-                    hipStream_t s) {
-  if (p.view.nrows == 0) return PDA_OK;
-  int rc = launch_ghosts(p, U, s);
-  if (rc) return rc;
-  if (p.view.dup) {
-    hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, J, p.nnz, 0.);
-    PDA_HIP_CHECK(hipGetLastError());
-  }
-  bool box_done = false;
-  if (p.use_box && p.box.valid && !p.view.dup && has_box_kernels(p)) {
-    rc = launch_jacobian_box(p, U, J, V, s);
-    if (rc == PDA_OK) box_done = true;
-    else if (rc != PDA_E_UNSUPPORTED) return rc;
-  }
-  const View v = generic_view(p, box_done);
+static int jac_generic(const View &v, const pda_problem_s &p, const double *U, double *V,
+                       double *J, hipStream_t s) {
   switch (p.family) {
     case PDA_EULER1D: return jac_dispatch<PhEuler1>(v, p.w, U, V, J, s);
     case PDA_EULER2D: return jac_dispatch<PhEuler2>(v, p.w, U, V, J, s);
@@ -414,6 +115,40 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
     case PDA_BURGERS2D: return jac_dispatch<PhBurgers>(v, p.w, U, V, J, s);
     default: set_error("jacobian: unsupported family"); return PDA_E_UNSUPPORTED;
   }
+}
+
+static bool box_velocity(const pda_problem_s &p) {
+  return p.use_box && p.box.valid && p.box.cells() > 0 && has_box_kernels(p);
+}
+static bool box_jacobian(const pda_problem_s &p) {
+  return box_velocity(p) && !p.view.dup && p.w == 1;
+}
+
+// With a structured box the box kernel and the generic kernel over the rows
+// outside the box run back to back on the caller's stream (disjoint rows);
+// a box launcher may take the outside rows into its own launch.
+int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, hipStream_t s) {
+  if (p.view.nrows == 0) return PDA_OK;
+  if (!box_velocity(p)) return vel_generic(generic_view(p, false), p, U, V, s);
+  int rc = p.h_rem_rows.empty() ? PDA_OK : vel_generic(generic_view(p, true), p, U, V, s);
+  if (rc) return rc;
+  return launch_velocity_box(p, U, V, s);
+}
+
+int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, double *V,
+                    hipStream_t s) {
+  if (p.view.nrows == 0) return PDA_OK;
+  if (!box_jacobian(p)) {
+    if (p.view.dup) {
+      hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, J, p.nnz, 0.);
+      PDA_HIP_CHECK(hipGetLastError());
+    }
+    return jac_generic(generic_view(p, false), p, U, V, J, s);
+  }
+  bool rest_done = false;
+  int rc = launch_jacobian_box(p, U, J, V, s, &rest_done);
+  if (rc || rest_done || p.h_rem_rows.empty()) return rc;
+  return jac_generic(generic_view(p, true), p, U, V, J, s);
 }
 
 int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64_t ncols,

@@ -137,6 +137,56 @@ __device__ __forceinline__ void weno5(const double *q, double &un, double &up) {
   }
 }
 
+// Same reconstruction with the normalised weights formed with one division:
+// alpha_k / sum(alpha) = N_k / sum(N) with N_k = d_k * prod_{j != k} D_j and
+// D_k = eps^2 + 2 eps B_k + B_k^2 (the reference's denominator).  Equal to
+// weno5() in exact arithmetic; the results differ by a few ulps.
+__device__ __forceinline__ double weno5_side_1div(double q0, double q1, double q2, double q3,
+                                                  double q4, double d0, double d1, double d2) {
+  constexpr double eps = 1.e-6, c1312 = 13. / 12., c16 = 1. / 6.;
+  const double p0 = (2. * q0 - 7. * q1 + 11. * q2) * c16;
+  const double p1 = (-q1 + 5. * q2 + 2. * q3) * c16;
+  const double p2 = (2. * q2 + 5. * q3 - q4) * c16;
+  const double t0 = q0 - 2. * q1 + q2, s0 = q0 - 4. * q1 + 3. * q2;
+  const double t1 = q1 - 2. * q2 + q3, s1 = q1 - q3;
+  const double t2 = q2 - 2. * q3 + q4, s2 = 3. * q2 - 4. * q3 + q4;
+  const double B0 = c1312 * t0 * t0 + 0.25 * s0 * s0;
+  const double B1 = c1312 * t1 * t1 + 0.25 * s1 * s1;
+  const double B2 = c1312 * t2 * t2 + 0.25 * s2 * s2;
+  const double D0 = eps * eps + 2. * eps * B0 + B0 * B0;
+  const double D1 = eps * eps + 2. * eps * B1 + B1 * B1;
+  const double D2 = eps * eps + 2. * eps * B2 + B2 * B2;
+  const double n0 = d0 * (D1 * D2), n1 = d1 * (D0 * D2), n2 = d2 * (D0 * D1);
+  return (n0 * p0 + n1 * p1 + n2 * p2) / (n0 + n1 + n2);
+}
+
+// q = {q_{i-2}, ..., q_{i+3}}; un, up: left and right states of face i+1/2
+__device__ __forceinline__ void weno5_1div(const double *q, double &un, double &up) {
+  un = weno5_side_1div(q[0], q[1], q[2], q[3], q[4], 0.1, 0.6, 0.3);
+  // mirrored stencil: the right state of the face is the left-biased
+  // reconstruction of the reversed sequence
+  up = weno5_side_1div(q[5], q[4], q[3], q[2], q[1], 0.1, 0.6, 0.3);
+}
+
+// WENO3 counterpart of weno5_1div (impl/weno3.hpp:56-132)
+__device__ __forceinline__ void weno3_1div(const double *q, double &un, double &up) {
+  constexpr double eps = 1.e-6, c13 = 1. / 3., c23 = 2. / 3.;
+  {
+    const double p0 = (-q[0] + 3. * q[1]) * 0.5, p1 = (q[1] + q[2]) * 0.5;
+    const double B0 = (q[0] - q[1]) * (q[0] - q[1]), B1 = (q[1] - q[2]) * (q[1] - q[2]);
+    const double D0 = eps * eps + 2. * eps * B0 + B0 * B0, D1 = eps * eps + 2. * eps * B1 + B1 * B1;
+    const double n0 = c13 * D1, n1 = c23 * D0;
+    un = (n0 * p0 + n1 * p1) / (n0 + n1);
+  }
+  {
+    const double p0 = (q[1] + q[2]) * 0.5, p1 = (3. * q[2] - q[3]) * 0.5;
+    const double B0 = (q[1] - q[2]) * (q[1] - q[2]), B1 = (q[2] - q[3]) * (q[2] - q[3]);
+    const double D0 = eps * eps + 2. * eps * B0 + B0 * B0, D1 = eps * eps + 2. * eps * B1 + B1 * B1;
+    const double n0 = c23 * D1, n1 = c13 * D0;
+    up = (n0 * p0 + n1 * p1) / (n0 + n1);
+  }
+}
+
 // one side of impl/weno5.hpp:180-434.  c = linear weights {d0,d1,d2};
 // P[k][i] = d p_k / d q_i; T/S = the two smoothness building blocks with
 // their gradients; returns value, writes grad[0..4]
@@ -242,17 +292,19 @@ __device__ __forceinline__ void euler_flux(const double *qL, const double *qR, d
   const double half = 0.5;
   double velL[DIM], velR[DIM];
   const double rL = qL[0], rR = qR[0];
+  // one reciprocal per side (the reference divides per component)
+  const double iL = 1. / (rL + kEs), iR = 1. / (rR + kEs);
   double v2L = 0., v2R = 0.;
   for (int d = 0; d < DIM; ++d) {
-    velL[d] = qL[1 + d] / (rL + kEs);
-    velR[d] = qR[1 + d] / (rR + kEs);
+    velL[d] = qL[1 + d] * iL;
+    velR[d] = qR[1 + d] * iR;
     v2L += velL[d] * velL[d];
     v2R += velR[d] * velR[d];
   }
   const double unL = velL[A], unR = velR[A];
   const double pL = (g - 1.) * (qL[M - 1] - half * rL * v2L);
   const double pR = (g - 1.) * (qR[M - 1] - half * rR * v2R);
-  const double HL = (qL[M - 1] + pL) / rL, HR = (qR[M - 1] + pR) / rR;
+  const double HL = (qL[M - 1] + pL) * iL, HR = (qR[M - 1] + pR) * iR;
   double FL[M], FR[M];
   FL[0] = rL * unL;
   FR[0] = rR * unR;
@@ -277,82 +329,131 @@ __device__ __forceinline__ void euler_flux(const double *qL, const double *qR, d
 
 // exact Jacobians of the Rusanov flux w.r.t. the left and right states,
 // including the derivative of the Roe-averaged wave speed,
-// impl/euler_rusanov_flux_jacobian_function.hpp:54-406.  JL, JR row-major MxM.
+// impl/euler_rusanov_flux_jacobian_function.hpp:54-406.
+//
+// Each side's M x M block has the form
+//   J_s = 1/2 A(q_s) +- 1/2 smax I + 1/2 grad_s (qL - qR)^T-outer,
+// i.e. J_s(i,j) = 1/2 A_s(i,j) + [i==j] dsm + 1/2 grad_s[j] dq[i], where A_s
+// is the physical flux Jacobian along axis A at the side's state.  EulerSide
+// keeps the factors (side velocity, |v|^2, total enthalpy, gradient of smax,
+// dq and the signed diagonal term); entries are formed on demand, so a
+// kernel can stream a block to memory without holding all M*M values.
+template <int DIM> struct EulerSide {
+  double vel[DIM], V2, H, grad[DIM + 2], dq[DIM + 2], dsm;
+};
+
 template <int DIM, int A>
-__device__ __forceinline__ void euler_fjac(const double *qL, const double *qR, double gamma,
-                                           double *JL, double *JR) {
+__device__ __forceinline__ void euler_sides(const double *qL, const double *qR, double gamma,
+                                            EulerSide<DIM> &L, EulerSide<DIM> &R) {
   constexpr int M = DIM + 2;
   const double half = 0.5, gm1 = gamma - 1.;
-  const double *Q[2] = {qL, qR};
-  double vel[2][DIM], V2[2], H[2], aa[2], un[2];
-  for (int s = 0; s < 2; ++s) {
-    const double r = Q[s][0];
-    V2[s] = 0.;
+  const double igm1 = 1. / gm1;
+  double aa[2];
+  {
+    const double iL = 1. / (qL[0] + kEs), iR = 1. / (qR[0] + kEs);
+    L.V2 = 0.;
+    R.V2 = 0.;
+#pragma unroll
     for (int d = 0; d < DIM; ++d) {
-      vel[s][d] = Q[s][1 + d] / (r + kEs);
-      V2[s] += vel[s][d] * vel[s][d];
+      L.vel[d] = qL[1 + d] * iL;
+      R.vel[d] = qR[1 + d] * iR;
+      L.V2 += L.vel[d] * L.vel[d];
+      R.V2 += R.vel[d] * R.vel[d];
     }
-    un[s] = vel[s][A];
-    const double p = gm1 * (Q[s][M - 1] - half * r * V2[s]);
-    H[s] = (Q[s][M - 1] + p) / r;
-    aa[s] = gm1 * (H[s] - half * V2[s]); // a^2 of the side
+    const double pL = gm1 * (qL[M - 1] - half * qL[0] * L.V2);
+    const double pR = gm1 * (qR[M - 1] - half * qR[0] * R.V2);
+    L.H = (qL[M - 1] + pL) * iL;
+    R.H = (qR[M - 1] + pR) * iR;
+    aa[0] = gm1 * (L.H - half * L.V2);
+    aa[1] = gm1 * (R.H - half * R.V2);
   }
   const double rL = qL[0], rR = qR[0];
   const double r = sqrt(rR * rL), RT = sqrt(rR / rL);
   const double inv = 1. / (1. + RT);
   double vR[DIM];
   double vr2 = 0.;
+#pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    vR[d] = (vel[0][d] + RT * vel[1][d]) * inv;
+    vR[d] = (L.vel[d] + RT * R.vel[d]) * inv;
     vr2 += vR[d] * vR[d];
   }
-  const double Hr = (H[0] + RT * H[1]) * inv;
+  const double Hr = (L.H + RT * R.H) * inv;
   const double a = sqrt(gm1 * (Hr - half * vr2));
+  const double ia = 1. / a;
   const double smax = (DIM == 1 ? fabs(vR[0]) : sqrt(vr2)) + a;
   const double VMagSqrRoe = vr2 + kEs;
   const double rs = 1. / sqrt(VMagSqrRoe);
   double rel[DIM];
+#pragma unroll
   for (int d = 0; d < DIM; ++d) rel[d] = vR[d] * rs;
-  double *JJ[2] = {JL, JR};
+#pragma unroll
   for (int s = 0; s < 2; ++s) {
-    const double rX = Q[s][0];
+    EulerSide<DIM> &S = s ? R : L;
+    const double rX = s ? rR : rL;
     double VV = 0.;
-    for (int d = 0; d < DIM; ++d) VV += vel[s][d] * vR[d];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) VV += S.vel[d] * vR[d];
     const double f = 1. / (rX + r);
-    double grad[M];
     double g0 = 0.;
-    for (int d = 0; d < DIM; ++d) g0 -= half * (vel[s][d] + vR[d]) * rel[d];
-    g0 += half * gm1 / a *
-          (half * (VMagSqrRoe + VV) + half * (H[s] - Hr) - aa[s] / gm1 + half * (gamma - 2.) * V2[s]);
-    grad[0] = f * g0;
-    for (int d = 0; d < DIM; ++d) grad[1 + d] = f * (rel[d] - half * (gm1 * (vR[d] + gm1 * vel[s][d])) / a);
-    grad[M - 1] = half * f * gamma * gm1 / a;
-    double *J = JJ[s];
-    const double uX = un[s];
-    // physical flux Jacobian (times 1/2) for normal e_A
-    J[0] = 0.;
-    for (int d = 0; d < DIM; ++d) J[1 + d] = (d == A ? half : 0.);
-    J[M - 1] = 0.;
-    for (int i = 0; i < DIM; ++i) {
-      double *Ji = J + (1 + i) * M;
-      Ji[0] = half * ((i == A ? half * gm1 * V2[s] : 0.) - vel[s][i] * uX);
-      for (int d = 0; d < DIM; ++d) {
-        double v = (d == A ? vel[s][i] : 0.) - gm1 * vel[s][d] * (i == A ? 1. : 0.);
-        if (d == i) v += uX;
-        Ji[1 + d] = half * v;
-      }
-      Ji[M - 1] = (i == A ? half * gm1 : 0.);
-    }
-    double *Je = J + (M - 1) * M;
-    Je[0] = half * ((half * gm1 * V2[s] - H[s]) * uX);
-    for (int d = 0; d < DIM; ++d) Je[1 + d] = half * ((d == A ? H[s] : 0.) - gm1 * vel[s][d] * uX);
-    Je[M - 1] = half * gamma * uX;
-    for (int i = 0; i < M; ++i) {
-      J[i * M + i] += (s == 0 ? half * smax : -half * smax);
-      const double dq = qL[i] - qR[i];
-      for (int j = 0; j < M; ++j) J[i * M + j] += half * grad[j] * dq;
-    }
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) g0 -= half * (S.vel[d] + vR[d]) * rel[d];
+    g0 += half * gm1 * ia *
+          (half * (VMagSqrRoe + VV) + half * (S.H - Hr) - aa[s] * igm1 + half * (gamma - 2.) * S.V2);
+    S.grad[0] = f * g0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) S.grad[1 + d] = f * (rel[d] - half * (gm1 * (vR[d] + gm1 * S.vel[d])) * ia);
+    S.grad[M - 1] = half * f * gamma * gm1 * ia;
+#pragma unroll
+    for (int i = 0; i < M; ++i) S.dq[i] = qL[i] - qR[i];
+    S.dsm = s ? -half * smax : half * smax;
   }
+}
+
+// entry (i, j) of one side's block; i and j are compile-time after unrolling
+template <int DIM, int A>
+__device__ __forceinline__ double euler_side_entry(const EulerSide<DIM> &S, double gamma, int i,
+                                                   int j) {
+  constexpr int M = DIM + 2;
+  const double half = 0.5, gm1 = gamma - 1.;
+  const double uX = S.vel[A];
+  double v;
+  if (i == 0) {
+    v = (j == 1 + A) ? half : 0.;
+  } else if (i <= DIM) {
+    const int ii = i - 1;
+    if (j == 0) {
+      v = half * ((ii == A ? half * gm1 * S.V2 : 0.) - S.vel[ii] * uX);
+    } else if (j <= DIM) {
+      const int d = j - 1;
+      double w = (d == A ? S.vel[ii] : 0.) - (ii == A ? gm1 * S.vel[d] : 0.);
+      if (d == ii) w += uX;
+      v = half * w;
+    } else {
+      v = (ii == A ? half * gm1 : 0.);
+    }
+  } else {
+    if (j == 0) v = half * ((half * gm1 * S.V2 - S.H) * uX);
+    else if (j <= DIM) v = half * ((j - 1 == A ? S.H : 0.) - gm1 * S.vel[j - 1] * uX);
+    else v = half * gamma * uX;
+  }
+  if (i == j) v += S.dsm;
+  return v + half * S.grad[j] * S.dq[i];
+}
+
+// JL, JR row-major M x M
+template <int DIM, int A>
+__device__ __forceinline__ void euler_fjac(const double *qL, const double *qR, double gamma,
+                                           double *JL, double *JR) {
+  constexpr int M = DIM + 2;
+  EulerSide<DIM> L, R;
+  euler_sides<DIM, A>(qL, qR, gamma, L, R);
+#pragma unroll
+  for (int i = 0; i < M; ++i)
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      JL[i * M + j] = euler_side_entry<DIM, A>(L, gamma, i, j);
+      JR[i * M + j] = euler_side_entry<DIM, A>(R, gamma, i, j);
+    }
 }
 
 // ------------------------------------------------------- shallow water

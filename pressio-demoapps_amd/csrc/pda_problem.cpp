@@ -341,7 +341,13 @@ int build(pda_problem_s &p) {
     for (int a = 0; a < 3; ++a) {
       p.box.n[a] = M.n[a];
       if (a >= dim) { p.box.lo[a] = 0; p.box.hi[a] = 1; continue; }
-      const int64_t lo = ((M.periodic_mask >> a) & 1) ? w : wm;
+      // rows within the mesh stencil of a non-periodic boundary are
+      // near-boundary rows, but where the scheme's own stencil stays in the
+      // grid (no ghost read, no BC factor) their Euler and shallow-water
+      // values equal the inner-row values; the Burgers viscous term is an
+      // inner-row term, so its box stays at the mesh stencil
+      const bool per = (M.periodic_mask >> a) & 1;
+      const int64_t lo = (per || p.family != PDA_BURGERS2D) ? w : wm;
       p.box.lo[a] = lo;
       p.box.hi[a] = M.n[a] - lo;
       if (p.box.hi[a] <= lo) ok = false;

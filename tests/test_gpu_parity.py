@@ -56,8 +56,10 @@ def test_velocity_and_jacobian_match_oracle(case, oracle, torch):
         assert cases.rel_err(Vg, Vo) <= TOL, cases.rel_err(Vg, Vo)
         assert cases.rel_err(V2g, V2o) <= TOL, cases.rel_err(V2g, V2o)
         assert cases.rel_err(Jg, Jo) <= TOL, cases.rel_err(Jg, Jo)
-        # jacobian() without V gives the same values
-        assert np.array_equal(Jg, Jvg)
+        # jacobian() without V gives the same values up to rounding: the two
+        # calls run separately compiled kernels, which may contract
+        # multiply-adds differently
+        assert cases.rel_err(Jvg, Jg) <= 1e-14, cases.rel_err(Jvg, Jg)
 
 
 @pytest.mark.parametrize("case", [c for c in cases.CASES if "riemann-Weno5" in c[0]
