@@ -29,6 +29,19 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def traffic_bytes(n):
+    """HBM bytes per launch of the Jacobian kernel as measured by rocprofv3
+    PMC counters (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy of known
+    size), read from the committed profile summary for this grid; None when
+    no measurement for this grid exists."""
+    path = os.path.join(ROOT, "profiles", "jacobian_fo_euler2d_riemann_%dx%d_pmc.json" % (n, n))
+    try:
+        with open(path) as f:
+            return json.load(f)["traffic_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(n_cpu, budget_s):
     """Reference CPU path (oracle/_ref, OpenMP) or the C restatement, timed on
     this host on an n_cpu x n_cpu sample of the same workload."""
@@ -86,6 +99,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="after timing, run one plain 256 MiB device copy (the reference "
+                         "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
@@ -155,6 +171,14 @@ def main():
         f"({ach_j:.0f} GB/s), step {ms_per_step*1e3:.1f} us, unit bytes {(bytes_v+bytes_j)/1e6:.0f} MB "
         f"-> {(bytes_v+bytes_j)/(wall/args.steps)/1e9:.0f} GB/s")
 
+    if args.copy_calib:
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copy calibration: 256 MiB read + 256 MiB written")
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -179,8 +203,9 @@ def main():
                                f"jacobian per perturbation", "cells": cells, "jacobian_nnz": nnz,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": ach_j, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach_j / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "first-order jacobian (k_jacobian)",
+                     "frac": ach_j / HBM_PEAK_GBS, "traffic": traffic_bytes(n),
+                     "kernel": "k_jac_fo_euler2d (first-order Euler 2D Jacobian, box cells and "
+                               "boundary ring in one launch; the whole jacobian() call)",
                      "bytes_per_launch": bytes_j, "avg_launch_us": t_j * 1e6},
         "unit_bytes_per_step": bytes_v + bytes_j,
         "hbm_frac_per_step": (bytes_v + bytes_j) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
