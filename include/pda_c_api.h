@@ -144,6 +144,26 @@ int pda_apply_jacobian(pda_problem p, const double *d_U, double t,
                        const double *d_B, int64_t ncols, int layout,
                        double *d_AJ, pda_stream stream);
 
+/* ---------------------------------------------------- explicit stepping */
+
+/* Explicit time integration of dU/dt = velocity(U) on the device with
+   pressio's stage formulas
+   (pressio/ode/impl/ode_explicit_stepper_without_mass_matrix.hpp: forward
+   Euler :168-187, SSPRK3 :230-281, RK4 :284-352), as used by the
+   reference's tests and by pressiodemoapps.advanceRK4/advanceSSP3
+   (pressiodemoapps/__init__.py:117-185).  Full meshes only; a stepper
+   must not outlive its problem. */
+enum { PDA_FORWARD_EULER = 0, PDA_SSPRK3 = 1, PDA_RK4 = 2,
+       PDA_RK2 = 3 /* advanceRK2, pressiodemoapps/__init__.py:90-114 */ };
+typedef struct pda_stepper_s *pda_stepper;
+int pda_stepper_create(pda_problem p, int scheme, pda_stepper *out);
+/* nsteps steps of size dt from time t0, in place on d_U.  use_graph != 0
+   records one step as a hipGraph (re-recorded when d_U or dt change) and
+   replays it per step on the stream. */
+int pda_stepper_advance(pda_stepper st, double *d_U, double t0, double dt,
+                        int64_t nsteps, int use_graph, pda_stream stream);
+void pda_stepper_destroy(pda_stepper st);
+
 /* Block view of the Jacobian (m x m blocks).  For each block-row r and block
    slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values
    start at d_Jvals[rowptr[r*m] + s*m] with row stride nblk[r]*m. */

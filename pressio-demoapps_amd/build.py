@@ -27,7 +27,8 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
-SOURCES = ["pda_mesh.cpp", "pda_problem.cpp", "pda_kernels.hip", "pda_box_kernels.hip"]
+SOURCES = ["pda_mesh.cpp", "pda_problem.cpp", "pda_kernels.hip", "pda_box_kernels.hip",
+           "pda_stepper.hip"]
 
 
 def _compile(src):

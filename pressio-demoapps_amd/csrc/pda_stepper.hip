@@ -1,0 +1,209 @@
+// pda_stepper.hip -- explicit time integration of dU/dt = velocity(U) on the
+// device: forward Euler, SSP Runge-Kutta 3 and classical Runge-Kutta 4 with
+// pressio's stage formulas
+// (tests_cpp/pressio/include/pressio/ode/impl/ode_explicit_stepper_without_mass_matrix.hpp:
+//  forward Euler :168-187, SSPRK3 :230-281, RK4 :284-352; the Python
+// module's advanceRK2/advanceRK4/advanceSSP3, pressiodemoapps/__init__.py:90-185).
+//
+// A step is a fixed sequence of velocity evaluations and stage updates on
+// the stepper's own buffers.  With graphs enabled it is captured once into a
+// hipGraph (on an internal stream) and replayed per step on the caller's
+// stream; the capture is redone when the state pointer or dt changes.  The
+// evaluation kernels use no memsets, so the captured step has only kernel
+// nodes.
+#include "pda_internal.hpp"
+
+namespace pda {
+namespace {
+
+// out[i] = c0*x0[i] + c1*x1[i] + ... (left to right, up to five terms)
+struct Lin {
+  const double *x[5];
+  double c[5];
+  int n;
+};
+
+// out may alias an input (element-wise read before write)
+__global__ void __launch_bounds__(256) k_lincomb(double *out, Lin L, int64_t len) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += stride) {
+    double v = L.c[0] * L.x[0][i];
+    for (int k = 1; k < L.n; ++k) v += L.c[k] * L.x[k][i];
+    out[i] = v;
+  }
+}
+
+int lincomb(double *out, std::initializer_list<std::pair<double, const double *>> terms, int64_t len,
+            hipStream_t s) {
+  Lin L{};
+  L.n = 0;
+  for (auto &t : terms) {
+    L.c[L.n] = t.first;
+    L.x[L.n] = t.second;
+    ++L.n;
+  }
+  const int64_t blocks = std::min<int64_t>((len + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_lincomb, dim3(unsigned(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, out, L,
+                     len);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+}  // namespace
+}  // namespace pda
+
+struct pda_stepper_s {
+  pda_problem_s *prob = nullptr;
+  int scheme = 0;
+  int64_t len = 0;
+  pda::DevBuf aux, k[4];
+  hipStream_t cap = nullptr;         // capture stream
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  const double *g_U = nullptr;       // arguments the captured step was recorded with
+  double g_dt = 0., g_t = 0.;
+  ~pda_stepper_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace {
+using namespace pda;
+
+// one step U <- step(U) at time t; buffers of the stepper
+int step_once(pda_stepper_s &st, double *U, double t, double dt, hipStream_t s) {
+  const pda_problem_s &p = *st.prob;
+  const int64_t n = st.len;
+  double *aux = st.aux.as<double>();
+  double *k1 = st.k[0].as<double>(), *k2 = st.k[1].as<double>();
+  double *k3 = st.k[2].as<double>(), *k4 = st.k[3].as<double>();
+  int rc;
+#define PDA_TRY(x) \
+  do {             \
+    rc = (x);      \
+    if (rc) return rc; \
+  } while (0)
+  switch (st.scheme) {
+    case PDA_FORWARD_EULER:
+      // y_{n+1} = y_n + dt f(y_n)
+      PDA_TRY(launch_velocity(p, U, t, k1, s));
+      PDA_TRY(lincomb(U, {{1., U}, {dt, k1}}, n, s));
+      break;
+    case PDA_SSPRK3: {
+      const double third = 1. / 3., twoThird = 2. / 3., quarter = 1. / 4., threeQuarter = 3. / 4.;
+      PDA_TRY(launch_velocity(p, U, t, k1, s));
+      PDA_TRY(lincomb(aux, {{1., U}, {dt, k1}}, n, s));                       // u1 = u + dt f(u)
+      PDA_TRY(launch_velocity(p, aux, t + dt, k1, s));
+      PDA_TRY(lincomb(aux, {{quarter, aux}, {threeQuarter, U}, {quarter * dt, k1}}, n, s));
+      PDA_TRY(launch_velocity(p, aux, t + dt / 2., k1, s));
+      PDA_TRY(lincomb(U, {{third, U}, {twoThird, aux}, {twoThird * dt, k1}}, n, s));
+      break;
+    }
+    case PDA_RK4: {
+      const double h2 = dt / 2., h6 = dt / 6., h3 = dt / 3.;
+      PDA_TRY(launch_velocity(p, U, t, k1, s));
+      PDA_TRY(lincomb(aux, {{1., U}, {h2, k1}}, n, s));
+      PDA_TRY(launch_velocity(p, aux, t + h2, k2, s));
+      PDA_TRY(lincomb(aux, {{1., U}, {h2, k2}}, n, s));
+      PDA_TRY(launch_velocity(p, aux, t + h2, k3, s));
+      PDA_TRY(lincomb(aux, {{1., U}, {dt, k3}}, n, s));
+      PDA_TRY(launch_velocity(p, aux, t + dt, k4, s));
+      PDA_TRY(lincomb(U, {{1., U}, {h6, k1}, {h3, k2}, {h3, k3}, {h6, k4}}, n, s));
+      break;
+    }
+    case PDA_RK2: {
+      // pressiodemoapps/__init__.py:90-114: k1 = dt f(y), k2 = dt f(y + k1),
+      // y += k2/2 + k1/2
+      PDA_TRY(launch_velocity(p, U, t, k1, s));
+      PDA_TRY(lincomb(aux, {{1., U}, {dt, k1}}, n, s));
+      PDA_TRY(launch_velocity(p, aux, t + dt, k2, s));
+      PDA_TRY(lincomb(U, {{1., U}, {0.5 * dt, k2}, {0.5 * dt, k1}}, n, s));
+      break;
+    }
+    default:
+      set_error("stepper: unknown scheme");
+      return PDA_E_INVALID_ARG;
+  }
+#undef PDA_TRY
+  return PDA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pda_stepper_create(pda_problem p, int scheme, pda_stepper *out) {
+  if (!p || !out || scheme < PDA_FORWARD_EULER || scheme > PDA_RK2) {
+    set_error("pda_stepper_create: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  if (p->mesh->n_sample != p->mesh->n_stencil) {
+    set_error("pda_stepper_create: explicit stepping needs a full mesh (sample == stencil mesh)");
+    return PDA_E_INVALID_ARG;
+  }
+  int rc = ensure_device(*p);
+  if (rc) return rc;
+  auto *st = new pda_stepper_s();
+  st->prob = p;
+  st->scheme = scheme;
+  st->len = p->mesh->n_stencil * p->m;
+  const size_t bytes = sizeof(double) * size_t(st->len);
+  hipError_t e = st->aux.alloc(bytes);
+  for (int i = 0; i < 4 && e == hipSuccess; ++i) e = st->k[i].alloc(bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&st->cap, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    set_error(std::string("pda_stepper_create: ") + hipGetErrorString(e));
+    delete st;
+    return PDA_E_HIP;
+  }
+  *out = st;
+  return PDA_OK;
+}
+
+void pda_stepper_destroy(pda_stepper st) { delete st; }
+
+int pda_stepper_advance(pda_stepper st, double *d_U, double t0, double dt, int64_t nsteps,
+                        int use_graph, pda_stream stream) {
+  if (!st || !d_U || nsteps < 0) {
+    set_error("pda_stepper_advance: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  hipStream_t s = hipStream_t(stream);
+  if (!use_graph) {
+    for (int64_t i = 0; i < nsteps; ++i) {
+      const int rc = step_once(*st, d_U, t0 + double(i) * dt, dt, s);
+      if (rc) return rc;
+    }
+    return PDA_OK;
+  }
+  // the problems' velocity does not depend on t, so one captured step
+  // serves every step; it is re-recorded when U or dt change
+  if (!st->exec || st->g_U != d_U || st->g_dt != dt) {
+    if (st->exec) PDA_HIP_CHECK(hipGraphExecDestroy(st->exec));
+    if (st->graph) PDA_HIP_CHECK(hipGraphDestroy(st->graph));
+    st->exec = nullptr;
+    st->graph = nullptr;
+    PDA_HIP_CHECK(hipStreamBeginCapture(st->cap, hipStreamCaptureModeThreadLocal));
+    const int rc = step_once(*st, d_U, t0, dt, st->cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(st->cap, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) {
+      set_error(std::string("pda_stepper_advance: capture: ") + hipGetErrorString(e));
+      return PDA_E_HIP;
+    }
+    st->graph = g;
+    PDA_HIP_CHECK(hipGraphInstantiate(&st->exec, st->graph, nullptr, nullptr, 0));
+    st->g_U = d_U;
+    st->g_dt = dt;
+  }
+  for (int64_t i = 0; i < nsteps; ++i) PDA_HIP_CHECK(hipGraphLaunch(st->exec, s));
+  return PDA_OK;
+}
+
+}  // extern "C"

@@ -62,6 +62,9 @@ def lib():
             "pda_jacobian": ([vp, vp, dbl, vp, vp, vp], i32),
             "pda_apply_jacobian": ([vp, vp, dbl, vp, i64, i32, vp, vp], i32),
             "pda_jacobian_blocks": ([vp, P(i32), P(vp), P(vp)], i32),
+            "pda_stepper_create": ([vp, i32, P(vp)], i32),
+            "pda_stepper_advance": ([vp, vp, dbl, dbl, i64, i32, vp], i32),
+            "pda_stepper_destroy": ([vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -427,3 +430,68 @@ def create_problem(mesh, problem_enum, scheme=InviscidFluxReconstruction.FirstOr
     if isinstance(ic_flag, dict):
         params, ic_flag = ic_flag, 0
     return Problem(mesh, problem_enum, scheme, ic_flag, params)
+
+
+# ------------------------------------------------------- explicit stepping
+class StepScheme(enum.IntEnum):
+    ForwardEuler = 0
+    SSPRK3 = 1
+    RK4 = 2
+    RK2 = 3
+
+
+class ExplicitStepper:
+    """Device time integrator for dU/dt = velocity(U) (pda_stepper_*):
+    pressio's forward Euler / SSPRK3 / RK4 stage formulas
+    (pressio/ode/impl/ode_explicit_stepper_without_mass_matrix.hpp:168-352)
+    and the module's RK2 (pressiodemoapps/__init__.py:90-114).  With
+    use_graph one step is recorded as a hipGraph and replayed."""
+
+    def __init__(self, problem, scheme):
+        self._p = problem
+        h = ctypes.c_void_p()
+        _check(lib().pda_stepper_create(problem.handle, int(scheme), ctypes.byref(h)))
+        self._h = h.value
+
+    def advance(self, state, dt, nsteps, start_time=0.0, use_graph=True):
+        _dev_tensor(state, self._p.totalDofStencilMesh(), "state")
+        _check(lib().pda_stepper_advance(self._h, _ptr(state), float(start_time), float(dt),
+                                         int(nsteps), 1 if use_graph else 0, _stream()))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pda_stepper_destroy(h)
+            self._h = None
+
+
+def _advance(scheme, appObj, state, dt, Nsteps, startTime, observer, use_graph):
+    st = ExplicitStepper(appObj, scheme)
+    if observer is None:
+        st.advance(state, dt, Nsteps, startTime, use_graph)
+        return
+    v = appObj.createRightHandSide()
+    t = startTime
+    for step in range(1, Nsteps + 1):
+        appObj.rightHandSide(state, t, v)
+        observer(step - 1, state, v)
+        st.advance(state, dt, 1, t, use_graph)
+        t += dt
+
+
+def advanceRK2(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showProgress=False,
+               use_graph=True):
+    """pressiodemoapps.advanceRK2 (pressiodemoapps/__init__.py:90-114), on the device."""
+    _advance(StepScheme.RK2, appObj, state, dt, Nsteps, startTime, observer, use_graph)
+
+
+def advanceRK4(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showProgress=False,
+               use_graph=True):
+    """pressiodemoapps.advanceRK4 (pressiodemoapps/__init__.py:117-152), on the device."""
+    _advance(StepScheme.RK4, appObj, state, dt, Nsteps, startTime, observer, use_graph)
+
+
+def advanceSSP3(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showProgress=False,
+                use_graph=True):
+    """pressiodemoapps.advanceSSP3 (pressiodemoapps/__init__.py:155-185), on the device."""
+    _advance(StepScheme.SSPRK3, appObj, state, dt, Nsteps, startTime, observer, use_graph)

@@ -87,3 +87,57 @@ def test_apply_jacobian_matches_csr_product(case, oracle, torch):
     prob.applyJacobian(Ud, b, 0.0, AJ)
     torch.cuda.synchronize()
     assert cases.rel_err(AJ.cpu().numpy(), ref[:, 0]) <= TOL
+
+
+SAMPLE_CASES = [c for c in cases.CASES if c[0].endswith(("FirstOrder", "Weno3", "Weno5"))]
+
+
+@pytest.mark.parametrize("case", SAMPLE_CASES, ids=[c[0] for c in SAMPLE_CASES])
+def test_sample_mesh_matches_oracle(case, oracle, torch):
+    """Sample/stencil mesh (meshing_scripts/create_sample_mesh.py): a seeded
+    random 20 % of the cells as rows, their stencil cells as columns."""
+    print(f"[gpu-parity-sample] {case[0]}", flush=True)
+    full, _ = cases.make(case)
+    rng = np.random.default_rng(11)
+    nsamp = max(4, full.sampleMeshSize() // 5)
+    gids = np.sort(rng.choice(full.sampleMeshSize(), size=nsamp, replace=False))
+    sm = pda.create_sample_mesh(full, gids)
+    _, enum, scheme, n, bounds, st, per, ic, params = case
+    prob = pda.create_problem(sm, enum, scheme, ic, params)
+    U = cases.perturbed_state(prob, 5)
+    Vg, V2g, Jg, Jvg, rp, ci = _run_gpu(prob, U, torch)
+    ms = oracle.from_mesh(sm)
+    pb = cases.oracle_problem(case)
+    Vo = oracle.velocity(ms, pb, U)
+    V2o, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, U)
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio)
+    print(f"[gpu-parity-sample] {case[0]}: rows {nsamp} V {cases.rel_err(Vg, Vo):.2e} "
+          f"V(J path) {cases.rel_err(V2g, V2o):.2e} J {cases.rel_err(Jg, Jo):.2e}", flush=True)
+    assert cases.rel_err(Vg, Vo) <= TOL
+    assert cases.rel_err(V2g, V2o) <= TOL
+    assert cases.rel_err(Jg, Jo) <= TOL
+    assert cases.rel_err(Jvg, Jg) <= 1e-14
+
+
+def test_sample_mesh_5pct_sedov3d(oracle, torch):
+    """3D Sedov with symmetry BCs on a 5 % seeded random sample mesh of a
+    24^3 grid, first-order and WENO3."""
+    for scheme in (pda.InviscidFluxReconstruction.FirstOrder, pda.InviscidFluxReconstruction.Weno3):
+        full = pda.create_structured_mesh((24, 24, 24), (0, 1.2, 0, 1.2, 0, 1.2), 5)
+        rng = np.random.default_rng(7)
+        gids = np.sort(rng.choice(full.sampleMeshSize(), size=full.sampleMeshSize() // 20,
+                                  replace=False))
+        sm = pda.create_sample_mesh(full, gids)
+        prob = pda.create_problem(sm, pda.Euler3d.SedovSymmetry, scheme, 0, None)
+        U = cases.perturbed_state(prob, 9)
+        Vg, V2g, Jg, Jvg, rp, ci = _run_gpu(prob, U, torch)
+        import pyoracle
+        pb = pyoracle.Oracle.problem(3, int(pda.Euler3d.SedovSymmetry), int(scheme), [1.4])
+        ms = oracle.from_mesh(sm)
+        Vo = oracle.velocity(ms, pb, U)
+        V2o, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, U)
+        assert np.array_equal(rp, rpo) and np.array_equal(ci, cio)
+        print(f"[gpu-parity-sample] sedov3d 5% {scheme.name}: V {cases.rel_err(Vg, Vo):.2e} "
+              f"J {cases.rel_err(Jg, Jo):.2e}", flush=True)
+        assert cases.rel_err(Vg, Vo) <= TOL and cases.rel_err(V2g, V2o) <= TOL
+        assert cases.rel_err(Jg, Jo) <= TOL

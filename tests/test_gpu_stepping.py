@@ -1,0 +1,120 @@
+"""Explicit time integration on the device vs the CPU oracle.
+
+Sod shock tube (1000 cells on [-0.5, 0.5], WENO5, SSPRK3, dt = 1e-4 to
+t = 0.2, i.e. 2000 steps): the device velocity is checked against the oracle
+on the device's own state at every step (1e-12), the graph-replayed stepper
+against the eagerly launched one at every step (bit-exact), and the final
+state against the oracle's own SSPRK3 run with pressio's stage formulas
+(tests_cpp/pressio/include/pressio/ode/impl/ode_explicit_stepper_without_mass_matrix.hpp:230-281).
+"""
+import numpy as np
+import pytest
+
+import cases
+import pressiodemoapps_amd as pda
+
+pytestmark = pytest.mark.gpu
+S = pda.InviscidFluxReconstruction
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no HIP device")
+    return _t
+
+
+def ssprk3_cpu(f, y, dt, n):
+    for _ in range(n):
+        a = y + dt * f(y)
+        a = 0.25 * a + 0.75 * y + (0.25 * dt) * f(a)
+        y = (1. / 3.) * y + (2. / 3.) * a + (2. / 3. * dt) * f(a)
+    return y
+
+
+def rk4_cpu(f, y, dt, n):
+    for _ in range(n):
+        k1 = f(y)
+        k2 = f(y + (dt / 2) * k1)
+        k3 = f(y + (dt / 2) * k2)
+        k4 = f(y + dt * k3)
+        y = y + (dt / 6) * k1 + (dt / 3) * k2 + (dt / 3) * k3 + (dt / 6) * k4
+    return y
+
+
+def test_sod1d_weno5_ssprk3_2000_steps(oracle, torch):
+    print("[gpu-step] sod1d weno5 ssprk3 1000 cells 2000 steps", flush=True)
+    mesh = pda.create_structured_mesh((1000,), (-0.5, 0.5), 7)
+    prob = pda.create_problem(mesh, pda.Euler1d.Sod, S.Weno5)
+    ms = oracle.from_mesh(mesh)
+    import pyoracle
+    pb = pyoracle.Oracle.problem(1, int(pda.Euler1d.Sod), int(S.Weno5), [1.4])
+    f = lambda U: oracle.velocity(ms, pb, U)  # noqa: E731
+    U0 = prob.initialCondition()
+    Ue, Ug = U0.clone(), U0.clone()
+    eager = pda.ExplicitStepper(prob, pda.StepScheme.SSPRK3)
+    graph = pda.ExplicitStepper(prob, pda.StepScheme.SSPRK3)
+    V = prob.createRightHandSide()
+    dt, nsteps = 1e-4, 2000
+    worst_v = 0.0
+    for k in range(nsteps):
+        prob.rightHandSide(Ue, k * dt, V)
+        torch.cuda.synchronize()
+        Uh = Ue.cpu().numpy()
+        ev = cases.rel_err(V.cpu().numpy(), f(Uh))
+        worst_v = max(worst_v, ev)
+        assert ev <= 1e-12, (k, ev)
+        eager.advance(Ue, dt, 1, k * dt, use_graph=False)
+        graph.advance(Ug, dt, 1, k * dt, use_graph=True)
+        torch.cuda.synchronize()
+        assert torch.equal(Ue, Ug), k
+    Ucpu = ssprk3_cpu(f, U0.cpu().numpy(), dt, nsteps)
+    err = cases.rel_err(Ug.cpu().numpy(), Ucpu)
+    print(f"[gpu-step] sod1d: worst per-step V {worst_v:.2e}, final state vs oracle run {err:.2e}",
+          flush=True)
+    assert err <= 1e-10
+
+
+@pytest.mark.parametrize("case", [c for c in cases.CASES if c[0] in (
+    "euler2d-periodic-Weno5", "swe-slipwall-Weno3", "burgers-periodic-Weno5",
+    "euler2d-riemann-FirstOrder", "euler3d-periodic-Weno3")], ids=lambda c: c[0])
+def test_rk4_graph_matches_eager_and_oracle(case, oracle, torch):
+    print(f"[gpu-step] rk4 {case[0]}", flush=True)
+    mesh, prob = cases.make(case)
+    dt, nsteps = 1e-3, 60
+    U0 = torch.from_numpy(cases.perturbed_state(prob, 2)).cuda()
+    Ue, Ug = U0.clone(), U0.clone()
+    pda.advanceRK4(prob, Ue, dt, nsteps, use_graph=False)
+    # graph replayed in chunks with eager work in between
+    st = pda.ExplicitStepper(prob, pda.StepScheme.RK4)
+    V = prob.createRightHandSide()
+    for chunk in (1, 19, 40):
+        st.advance(Ug, dt, chunk, use_graph=True)
+        prob.rightHandSide(Ug, 0.0, V)
+        torch.cuda.synchronize()
+    assert torch.equal(Ue, Ug)
+    ms = oracle.from_mesh(mesh)
+    pb = cases.oracle_problem(case)
+    Ucpu = rk4_cpu(lambda U: oracle.velocity(ms, pb, U), U0.cpu().numpy(), dt, nsteps)
+    err = cases.rel_err(Ug.cpu().numpy(), Ucpu)
+    print(f"[gpu-step] {case[0]}: final state vs oracle run {err:.2e}", flush=True)
+    assert err <= 1e-11
+
+
+def test_rk2_and_observer(oracle, torch):
+    mesh, prob = cases.make([c for c in cases.CASES if c[0] == "euler1d-periodic-Weno3"][0])
+    U = prob.initialCondition()
+    seen = []
+    pda.advanceRK2(prob, U, 1e-3, 5, observer=lambda k, s, v: seen.append(k))
+    assert seen == [0, 1, 2, 3, 4]
+    ms = oracle.from_mesh(mesh)
+    pb = cases.oracle_problem([c for c in cases.CASES if c[0] == "euler1d-periodic-Weno3"][0])
+    f = lambda y: oracle.velocity(ms, pb, y)  # noqa: E731
+    y = prob.initialCondition().cpu().numpy()
+    for _ in range(5):
+        k1 = 1e-3 * f(y)
+        k2 = 1e-3 * f(y + k1)
+        y = y + k2 * 0.5 + k1 * 0.5
+    torch.cuda.synchronize()
+    assert cases.rel_err(U.cpu().numpy(), y) <= 1e-12
