@@ -164,6 +164,47 @@ int pda_stepper_advance(pda_stepper st, double *d_U, double t0, double dt,
                         int64_t nsteps, int use_graph, pda_stream stream);
 void pda_stepper_destroy(pda_stepper st);
 
+/* ---------------------------------------------------- implicit stepping */
+
+/* Implicit time stepping y_n -> y_{n+1} with pressio's discrete residuals
+   and Jacobians (pressio/ode/impl/ode_implicit_discrete_residual.hpp:112-240,
+   ode_implicit_discrete_jacobian.hpp:60-121):
+     BDF1: R = y - y_n - dt f(y),                        J = I - dt df/dy
+     BDF2: R = y - 4/3 y_n + 1/3 y_{n-1} - 2/3 dt f(y),   J = I - 2/3 dt df/dy
+     CN:   R = y - y_n - dt/2 (f(y) + f(y_n)),            J = I - dt/2 df/dy
+   solved by Newton on the exact Jacobian with a device BiCGSTAB
+   (block-Jacobi preconditioner).  Full meshes only; an implicit stepper
+   must not outlive its problem. */
+enum { PDA_BDF1 = 1, PDA_BDF2 = 2, PDA_CRANK_NICOLSON = 3 };
+/* PDA_STOP_CORRECTION: pressio's default rule, stop when ||dx||_2 < tol
+   without applying that correction
+   (pressio/solvers_nonlinear/impl/nonlinear_least_squares.hpp:70-84);
+   PDA_STOP_RESIDUAL: stop when ||R(y)||_2 <= tol. */
+enum { PDA_STOP_CORRECTION = 0, PDA_STOP_RESIDUAL = 1 };
+typedef struct {
+  int newton_iters;   /* Newton iterations (Jacobian evaluations with a solve) */
+  int linear_iters;   /* BiCGSTAB iterations summed over the step */
+  int converged;
+  double residual_norm;            /* ||R||_2 at the last evaluation */
+  double correction_norm;          /* ||dx||_2 of the last solve */
+  double last_linear_rel_residual; /* ||b - A dx|| / ||b|| of the last solve */
+} pda_implicit_stats;
+typedef struct pda_implicit_s *pda_implicit;
+int pda_implicit_create(pda_problem p, int scheme, pda_implicit *out);
+/* defaults: PDA_STOP_CORRECTION, tol 1e-6, 100 Newton iterations,
+   linear relative tolerance 1e-14, linear iteration cap 2*unknowns (0) */
+int pda_implicit_set_options(pda_implicit ip, int stop_rule, double tol, int max_newton,
+                             double lin_rtol, int lin_max);
+/* One step from time t: d_U holds y_n on entry and y_{n+1} on return.
+   d_Uprev = y_{n-1} for BDF2 (NULL: this step is BDF1, as pressio's first
+   BDF2 step).  d_guess: Newton's initial iterate (NULL: y_n, pressio's
+   choice).  Synchronizes the stream (the Newton loop reads norms).
+   Returns PDA_E_STATE when Newton does not converge. */
+int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev,
+                      const double *d_guess, double t, double dt, pda_stream stream,
+                      pda_implicit_stats *stats);
+void pda_implicit_destroy(pda_implicit ip);
+
 /* Block view of the Jacobian (m x m blocks).  For each block-row r and block
    slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values
    start at d_Jvals[rowptr[r*m] + s*m] with row stride nblk[r]*m. */

@@ -1,0 +1,557 @@
+// pda_implicit.hip -- implicit time stepping on the device: BDF1, BDF2 and
+// Crank-Nicolson residuals and Jacobians with pressio's coefficients
+// (tests_cpp/pressio/include/pressio/ode/impl/ode_implicit_discrete_residual.hpp:112-168,
+//  ode_implicit_discrete_jacobian.hpp:60-121), solved by Newton's method on
+// the exact Jacobian of the problem (pressio's Newton loop,
+// pressio/solvers_nonlinear/impl/nonlinear_least_squares.hpp:14-118) with a
+// device BiCGSTAB (the iteration of Eigen's BiCGSTAB, which pressio's
+// default linear solver wraps, solvers_linear_traits.hpp:95-105) and a
+// block-Jacobi preconditioner (the m x m diagonal blocks, inverted per cell).
+//
+// Stop rules:
+//   PDA_STOP_CORRECTION -- pressio's default: stop when the l2 norm of the
+//     Newton correction is below tol, without applying that correction
+//     (nonlinear_least_squares.hpp:70-84);
+//   PDA_STOP_RESIDUAL   -- stop when ||R(y)||_2 <= tol (the correction that
+//     reached it applied).
+// The BiCGSTAB scalars live in device memory (no host round trip per dot
+// product); the host polls the convergence flag every few iterations.
+#include <cmath>
+
+#include "pda_internal.hpp"
+
+namespace pda {
+namespace {
+
+constexpr int TB = 256; // threads per block of the vector kernels
+
+// scalar slots of the Krylov solver
+enum {
+  S_RHO = 0, S_RHO_OLD, S_ALPHA, S_W, S_BETA, S_R0V, S_TS, S_TT, S_RR, S_BB, S_TOL2, S_DONE,
+  S_NITER, S_COUNT
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  __syncthreads();
+  if (l == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < int(blockDim.x >> 6); ++i) s += sh[i];
+  return s; // valid in thread 0
+}
+
+// partial dot products (a.b, c.d) per block; part[2*blk], part[2*blk+1]
+__global__ void __launch_bounds__(TB) k_dot2(const double *__restrict__ a, const double *__restrict__ b,
+                                              const double *__restrict__ c, const double *__restrict__ d,
+                                              int64_t n, double *__restrict__ part) {
+  __shared__ double sh[TB / 64];
+  double s1 = 0., s2 = 0.;
+  for (int64_t i = int64_t(blockIdx.x) * TB + threadIdx.x; i < n; i += int64_t(gridDim.x) * TB) {
+    s1 += a[i] * b[i];
+    if (c) s2 += c[i] * d[i];
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// y = A x (scalar CSR, one thread per row); optional partial dots
+// (d1 . y) and (y . y) for the Krylov scalars
+__global__ void __launch_bounds__(TB) k_spmv(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                              const double *__restrict__ A, const double *__restrict__ x,
+                                              double *__restrict__ y, int64_t n,
+                                              const double *__restrict__ d1, int want_yy,
+                                              double *__restrict__ part, const double *__restrict__ scal) {
+  __shared__ double sh[TB / 64];
+  const int64_t i = int64_t(blockIdx.x) * TB + threadIdx.x;
+  const bool done = scal && scal[S_DONE] != 0.;
+  double s1 = 0., s2 = 0.;
+  if (i < n && !done) {
+    double acc = 0.;
+    for (int64_t j = rp[i]; j < rp[i + 1]; ++j) acc += A[j] * x[ci[j]];
+    y[i] = acc;
+    if (d1) s1 = d1[i] * acc;
+    if (want_yy) s2 = acc * acc;
+  }
+  if (part) {
+    s1 = block_sum(s1, sh);
+    s2 = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = s1;
+      part[2 * blockIdx.x + 1] = s2;
+    }
+  }
+}
+
+// Reduce the block partials (deterministic order) and advance the solver's
+// scalars.  stage 0: after r changed (rr, rho, beta, convergence);
+// 1: alpha = rho / (r0.v); 2: w = (t.s)/(t.t); 3: initial (bb, tol2).
+__global__ void __launch_bounds__(TB) k_finish(const double *__restrict__ part, int nblk, int stage,
+                                                double *__restrict__ scal, double rtol) {
+  __shared__ double sh[TB / 64];
+  double s1 = 0., s2 = 0.;
+  for (int i = threadIdx.x; i < nblk; i += TB) {
+    s1 += part[2 * i];
+    s2 += part[2 * i + 1];
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x != 0) return;
+  if (stage == 3) {
+    // s1 = r0.r = b.b (x0 = 0)
+    scal[S_BB] = s1;
+    scal[S_TOL2] = rtol * rtol * s1;
+    scal[S_RHO] = 1.;
+    scal[S_ALPHA] = 1.;
+    scal[S_W] = 1.;
+    scal[S_NITER] = 0.;
+    scal[S_DONE] = (s1 == 0.) ? 1. : 0.;
+    return;
+  }
+  if (scal[S_DONE] != 0.) return;
+  if (stage == 0) {
+    // s1 = r0.r, s2 = r.r
+    scal[S_RR] = s2;
+    if (s2 <= scal[S_TOL2]) {
+      scal[S_DONE] = 1.;
+      return;
+    }
+    scal[S_RHO_OLD] = scal[S_RHO];
+    scal[S_RHO] = s1;
+    scal[S_BETA] = (s1 / scal[S_RHO_OLD]) * (scal[S_ALPHA] / scal[S_W]);
+    scal[S_NITER] += 1.;
+  } else if (stage == 1) {
+    scal[S_R0V] = s1;
+    scal[S_ALPHA] = scal[S_RHO] / s1;
+  } else {
+    scal[S_TS] = s1;
+    scal[S_TT] = s2;
+    scal[S_W] = s2 > 0. ? s1 / s2 : 0.;
+  }
+}
+
+// Inverse of the m x m diagonal block of every cell (Gauss-Jordan with
+// partial pivoting); the block's entries are the m consecutive values of
+// each of the cell's scalar rows starting at column cell*m.
+template <int M>
+__global__ void __launch_bounds__(TB) k_block_inv(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                   const double *__restrict__ A, int64_t ncell,
+                                                   double *__restrict__ Minv) {
+  const int64_t c = int64_t(blockIdx.x) * TB + threadIdx.x;
+  if (c >= ncell) return;
+  double a[M][2 * M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int64_t row = c * M + k, end = rp[row + 1];
+    int64_t pos = rp[row];
+    while (pos < end && ci[pos] != int32_t(c * M)) ++pos;
+    const bool found = pos + M <= end;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      a[k][j] = found ? A[pos + j] : (j == k ? 1. : 0.);
+      a[k][M + j] = (j == k) ? 1. : 0.;
+    }
+  }
+#pragma unroll
+  for (int col = 0; col < M; ++col) {
+    int piv = col;
+    double best = fabs(a[col][col]);
+#pragma unroll
+    for (int r = col + 1; r < M; ++r)
+      if (fabs(a[r][col]) > best) {
+        best = fabs(a[r][col]);
+        piv = r;
+      }
+#pragma unroll
+    for (int r = col; r < M; ++r)
+      if (r == piv && r != col)
+#pragma unroll
+        for (int j = 0; j < 2 * M; ++j) {
+          const double t = a[col][j];
+          a[col][j] = a[r][j];
+          a[r][j] = t;
+        }
+    const double inv = 1. / a[col][col];
+#pragma unroll
+    for (int j = 0; j < 2 * M; ++j) a[col][j] *= inv;
+#pragma unroll
+    for (int r = 0; r < M; ++r) {
+      if (r == col) continue;
+      const double f = a[r][col];
+#pragma unroll
+      for (int j = 0; j < 2 * M; ++j) a[r][j] -= f * a[col][j];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k)
+#pragma unroll
+    for (int j = 0; j < M; ++j) Minv[(c * M + k) * M + j] = a[k][M + j];
+}
+
+// p = r + beta (p - w v); y = Minv p          (per cell)
+template <int M>
+__global__ void __launch_bounds__(TB) k_p_prec(const double *__restrict__ r, double *__restrict__ p,
+                                                const double *__restrict__ v, const double *__restrict__ Minv,
+                                                double *__restrict__ y, int64_t ncell,
+                                                const double *__restrict__ scal, int first) {
+  const int64_t c = int64_t(blockIdx.x) * TB + threadIdx.x;
+  if (c >= ncell || scal[S_DONE] != 0.) return;
+  const double beta = scal[S_BETA], w = scal[S_W];
+  double pc[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int64_t i = c * M + k;
+    pc[k] = first ? r[i] : r[i] + beta * (p[i] - w * v[i]);
+    p[i] = pc[k];
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    double s = 0.;
+#pragma unroll
+    for (int j = 0; j < M; ++j) s += Minv[(c * M + k) * M + j] * pc[j];
+    y[c * M + k] = s;
+  }
+}
+
+// s = r - alpha v; z = Minv s                 (per cell)
+template <int M>
+__global__ void __launch_bounds__(TB) k_s_prec(const double *__restrict__ r, const double *__restrict__ v,
+                                                const double *__restrict__ Minv, double *__restrict__ s,
+                                                double *__restrict__ z, int64_t ncell,
+                                                const double *__restrict__ scal) {
+  const int64_t c = int64_t(blockIdx.x) * TB + threadIdx.x;
+  if (c >= ncell || scal[S_DONE] != 0.) return;
+  const double alpha = scal[S_ALPHA];
+  double sc[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int64_t i = c * M + k;
+    sc[k] = r[i] - alpha * v[i];
+    s[i] = sc[k];
+  }
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    double t = 0.;
+#pragma unroll
+    for (int j = 0; j < M; ++j) t += Minv[(c * M + k) * M + j] * sc[j];
+    z[c * M + k] = t;
+  }
+}
+
+// x += alpha y + w z; r = s - w t; partial dots (r0.r, r.r)
+__global__ void __launch_bounds__(TB) k_xr(double *__restrict__ x, double *__restrict__ r,
+                                            const double *__restrict__ y, const double *__restrict__ z,
+                                            const double *__restrict__ s, const double *__restrict__ t,
+                                            const double *__restrict__ r0, int64_t n,
+                                            double *__restrict__ part, const double *__restrict__ scal) {
+  __shared__ double sh[TB / 64];
+  const int64_t i = int64_t(blockIdx.x) * TB + threadIdx.x;
+  const bool done = scal[S_DONE] != 0.;
+  double s1 = 0., s2 = 0.;
+  if (i < n && !done) {
+    const double alpha = scal[S_ALPHA], w = scal[S_W];
+    x[i] += alpha * y[i] + w * z[i];
+    const double ri = s[i] - w * t[i];
+    r[i] = ri;
+    s1 = r0[i] * ri;
+    s2 = ri * ri;
+  }
+  s1 = block_sum(s1, sh);
+  s2 = block_sum(s2, sh);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = s1;
+    part[2 * blockIdx.x + 1] = s2;
+  }
+}
+
+// A = cf * J + cnp1 * I in place (full meshes: the diagonal of scalar row i
+// is column i)
+__global__ void __launch_bounds__(TB) k_discrete_jac(const int64_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                      double *__restrict__ A, int64_t n, double cf,
+                                                      double cnp1) {
+  const int64_t i = int64_t(blockIdx.x) * TB + threadIdx.x;
+  if (i >= n) return;
+  for (int64_t j = rp[i]; j < rp[i + 1]; ++j) {
+    double v = cf * A[j];
+    if (ci[j] == int32_t(i)) v += cnp1;
+    A[j] = v;
+  }
+}
+
+// out = sum_k c_k x_k (up to 4 terms, null x skipped)
+__global__ void __launch_bounds__(TB) k_comb(double *out, const double *x0, double c0, const double *x1,
+                                              double c1, const double *x2, double c2, const double *x3,
+                                              double c3, int64_t n) {
+  for (int64_t i = int64_t(blockIdx.x) * TB + threadIdx.x; i < n; i += int64_t(gridDim.x) * TB) {
+    double v = c0 * x0[i];
+    if (x1) v += c1 * x1[i];
+    if (x2) v += c2 * x2[i];
+    if (x3) v += c3 * x3[i];
+    out[i] = v;
+  }
+}
+
+unsigned grid_of(int64_t n) { return unsigned(std::max<int64_t>(1, (n + TB - 1) / TB)); }
+unsigned grid_strided(int64_t n) { return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + TB - 1) / TB, 2048))); }
+
+}  // namespace
+}  // namespace pda
+
+struct pda_implicit_s {
+  pda_problem_s *prob = nullptr;
+  int scheme = 0;
+  int64_t n = 0, ncell = 0;
+  int stop = PDA_STOP_CORRECTION;
+  double tol = 1e-6;
+  int max_newton = 100;
+  double lin_rtol = 1e-14;
+  int lin_max = 0;          // 0: 2 * unknowns (Eigen's default)
+  int check_every = 4;      // iterations between convergence polls
+  pda::DevBuf J, F, Fn, R, Y, Yn, dx, r, r0, p, v, s, t, y, z, Minv, part, scal;
+  int64_t nblk = 0;
+  pda_implicit_stats last{};
+};
+
+namespace {
+using namespace pda;
+
+template <int M>
+int bicgstab_m(pda_implicit_s &ip, const double *b, double *x, hipStream_t st, int *iters,
+               double *rel_res) {
+  pda_problem_s &p = *ip.prob;
+  const int64_t n = ip.n, nc = ip.ncell;
+  const int64_t *rp = p.d_rowptr.as<int64_t>();
+  const int32_t *ci = p.d_colidx.as<int32_t>();
+  const double *A = ip.J.as<double>();
+  double *r = ip.r.as<double>(), *r0 = ip.r0.as<double>(), *pp = ip.p.as<double>();
+  double *v = ip.v.as<double>(), *s = ip.s.as<double>(), *t = ip.t.as<double>();
+  double *y = ip.y.as<double>(), *z = ip.z.as<double>(), *Mi = ip.Minv.as<double>();
+  double *part = ip.part.as<double>(), *scal = ip.scal.as<double>();
+  const unsigned gv = grid_of(n), gc = grid_of(nc);
+  const int nb = int(gv);
+  // preconditioner
+  hipLaunchKernelGGL((k_block_inv<M>), dim3(gc), dim3(TB), 0, st, rp, ci, A, nc, Mi);
+  // x = 0, r = r0 = b
+  hipLaunchKernelGGL(k_comb, dim3(grid_strided(n)), dim3(TB), 0, st, x, b, 0., nullptr, 0., nullptr, 0.,
+                     nullptr, 0., n);
+  hipLaunchKernelGGL(k_comb, dim3(grid_strided(n)), dim3(TB), 0, st, r, b, 1., nullptr, 0., nullptr, 0.,
+                     nullptr, 0., n);
+  hipLaunchKernelGGL(k_comb, dim3(grid_strided(n)), dim3(TB), 0, st, r0, b, 1., nullptr, 0., nullptr, 0.,
+                     nullptr, 0., n);
+  hipLaunchKernelGGL(k_dot2, dim3(gv), dim3(TB), 0, st, r0, r, r, r, n, part);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 3, scal, ip.lin_rtol);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 0, scal, ip.lin_rtol);
+  PDA_HIP_CHECK(hipGetLastError());
+  const int64_t maxit = ip.lin_max > 0 ? ip.lin_max : 2 * n;
+  double h[S_COUNT];
+  for (int64_t it = 0; it < maxit; ++it) {
+    hipLaunchKernelGGL((k_p_prec<M>), dim3(gc), dim3(TB), 0, st, r, pp, v, Mi, y, nc, scal, it == 0 ? 1 : 0);
+    hipLaunchKernelGGL(k_spmv, dim3(gv), dim3(TB), 0, st, rp, ci, A, y, v, n, r0, 0, part, scal);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 1, scal, ip.lin_rtol);
+    hipLaunchKernelGGL((k_s_prec<M>), dim3(gc), dim3(TB), 0, st, r, v, Mi, s, z, nc, scal);
+    hipLaunchKernelGGL(k_spmv, dim3(gv), dim3(TB), 0, st, rp, ci, A, z, t, n, s, 1, part, scal);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 2, scal, ip.lin_rtol);
+    hipLaunchKernelGGL(k_xr, dim3(gv), dim3(TB), 0, st, x, r, y, z, s, t, r0, n, part, scal);
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 0, scal, ip.lin_rtol);
+    PDA_HIP_CHECK(hipGetLastError());
+    if ((it + 1) % ip.check_every == 0 || it + 1 == maxit) {
+      PDA_HIP_CHECK(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+      PDA_HIP_CHECK(hipStreamSynchronize(st));
+      if (h[S_DONE] != 0.) break;
+    }
+  }
+  PDA_HIP_CHECK(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+  PDA_HIP_CHECK(hipStreamSynchronize(st));
+  *iters = int(h[S_NITER]);
+  *rel_res = h[S_BB] > 0. ? std::sqrt(h[S_RR] / h[S_BB]) : 0.;
+  // a solve that stops at the iteration limit is reported through rel_res;
+  // the Newton loop's own stop rule decides
+  return PDA_OK;
+}
+
+int bicgstab(pda_implicit_s &ip, const double *b, double *x, hipStream_t st, int *iters, double *rel) {
+  switch (ip.prob->m) {
+    case 1: return bicgstab_m<1>(ip, b, x, st, iters, rel);
+    case 2: return bicgstab_m<2>(ip, b, x, st, iters, rel);
+    case 3: return bicgstab_m<3>(ip, b, x, st, iters, rel);
+    case 4: return bicgstab_m<4>(ip, b, x, st, iters, rel);
+    case 5: return bicgstab_m<5>(ip, b, x, st, iters, rel);
+  }
+  set_error("implicit: unsupported block size");
+  return PDA_E_UNSUPPORTED;
+}
+
+double norm2(pda_implicit_s &ip, const double *a, hipStream_t st) {
+  const unsigned g = grid_strided(ip.n);
+  double *part = ip.part.as<double>();
+  hipLaunchKernelGGL(k_dot2, dim3(g), dim3(TB), 0, st, a, a, nullptr, nullptr, ip.n, part);
+  std::vector<double> h(2 * g);
+  if (hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return NAN;
+  if (hipStreamSynchronize(st) != hipSuccess) return NAN;
+  double s = 0.;
+  for (unsigned i = 0; i < g; ++i) s += h[2 * i];
+  return std::sqrt(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pda_implicit_create(pda_problem p, int scheme, pda_implicit *out) {
+  if (!p || !out || (scheme != PDA_BDF1 && scheme != PDA_BDF2 && scheme != PDA_CRANK_NICOLSON)) {
+    set_error("pda_implicit_create: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  if (p->mesh->n_sample != p->mesh->n_stencil) {
+    set_error("pda_implicit_create: implicit stepping needs a full mesh (sample == stencil mesh)");
+    return PDA_E_INVALID_ARG;
+  }
+  if (p->m > 5) {
+    set_error("pda_implicit_create: unsupported block size");
+    return PDA_E_UNSUPPORTED;
+  }
+  int rc = ensure_device(*p);
+  if (rc) return rc;
+  // the Newton matrix and the Krylov solver work on the scalar CSR pattern
+  rc = ensure_pattern_device(*p);
+  if (rc) return rc;
+  if (!p->d_rowptr.ptr || !p->d_colidx.ptr) {
+    set_error("pda_implicit_create: Jacobian pattern not on the device");
+    return PDA_E_STATE;
+  }
+  auto *ip = new pda_implicit_s();
+  ip->prob = p;
+  ip->scheme = scheme;
+  ip->n = p->mesh->n_stencil * p->m;
+  ip->ncell = p->mesh->n_stencil;
+  const size_t vb = sizeof(double) * size_t(ip->n);
+  hipError_t e = ip->J.alloc(sizeof(double) * size_t(p->nnz));
+  pda::DevBuf *vecs[] = {&ip->F, &ip->Fn, &ip->R, &ip->Y, &ip->Yn, &ip->dx, &ip->r, &ip->r0,
+                         &ip->p, &ip->v, &ip->s, &ip->t, &ip->y, &ip->z};
+  for (auto *d : vecs)
+    if (e == hipSuccess) e = d->alloc(vb);
+  if (e == hipSuccess) e = ip->Minv.alloc(sizeof(double) * size_t(ip->ncell) * p->m * p->m);
+  ip->nblk = pda::grid_of(ip->n);
+  if (e == hipSuccess) e = ip->part.alloc(sizeof(double) * 2 * size_t(std::max<int64_t>(ip->nblk, 2048)));
+  if (e == hipSuccess) e = ip->scal.alloc(sizeof(double) * pda::S_COUNT);
+  if (e != hipSuccess) {
+    set_error(std::string("pda_implicit_create: ") + hipGetErrorString(e));
+    delete ip;
+    return PDA_E_HIP;
+  }
+  *out = ip;
+  return PDA_OK;
+}
+
+void pda_implicit_destroy(pda_implicit ip) { delete ip; }
+
+int pda_implicit_set_options(pda_implicit ip, int stop_rule, double tol, int max_newton,
+                             double lin_rtol, int lin_max) {
+  if (!ip || (stop_rule != PDA_STOP_CORRECTION && stop_rule != PDA_STOP_RESIDUAL) || !(tol > 0.) ||
+      max_newton < 1 || !(lin_rtol > 0.) || lin_max < 0) {
+    set_error("pda_implicit_set_options: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  ip->stop = stop_rule;
+  ip->tol = tol;
+  ip->max_newton = max_newton;
+  ip->lin_rtol = lin_rtol;
+  ip->lin_max = lin_max;
+  return PDA_OK;
+}
+
+int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev, const double *d_guess,
+                      double t, double dt, pda_stream stream, pda_implicit_stats *stats) {
+  if (!ip || !d_U || !(dt > 0.)) {
+    set_error("pda_implicit_step: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  hipStream_t st = hipStream_t(stream);
+  pda_problem_s &p = *ip->prob;
+  const int64_t n = ip->n;
+  if (!p.d_rowptr.ptr || !p.d_colidx.ptr || !p.on_device) {
+    set_error("pda_implicit_step: problem tables not on the device");
+    return PDA_E_STATE;
+  }
+  double *J = ip->J.as<double>(), *F = ip->F.as<double>(), *R = ip->R.as<double>();
+  double *Y = ip->Y.as<double>(), *Yn = ip->Yn.as<double>(), *dx = ip->dx.as<double>();
+  double *Fn = ip->Fn.as<double>();
+  const unsigned gs = grid_strided(n);
+  // residual coefficients R = cnp1 y + cn y_n + cnm1 y_{n-1} + cf f(y) [+ cfn f(y_n)]
+  int scheme = ip->scheme;
+  if (scheme == PDA_BDF2 && !d_Uprev) scheme = PDA_BDF1; // first BDF2 step is BDF1
+  double cnp1 = 1., cn = -1., cnm1 = 0., cf = -dt, cfn = 0.;
+  if (scheme == PDA_BDF2) {
+    cn = -4. / 3.;
+    cnm1 = 1. / 3.;
+    cf = -(2. / 3.) * dt;
+  } else if (scheme == PDA_CRANK_NICOLSON) {
+    cf = -0.5 * dt;
+    cfn = -0.5 * dt;
+  }
+  PDA_HIP_CHECK(hipMemcpyAsync(Yn, d_U, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  PDA_HIP_CHECK(hipMemcpyAsync(Y, d_guess ? d_guess : d_U, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  int rc;
+  if (scheme == PDA_CRANK_NICOLSON) {
+    rc = launch_velocity(p, Yn, t, Fn, st);
+    if (rc) return rc;
+  }
+  pda_implicit_stats S{};
+  S.converged = 0;
+  for (int it = 0; it < ip->max_newton; ++it) {
+    // f(y) and df/dy at the iterate (the Jacobian path's velocity, as
+    // pressio's residual-and-Jacobian evaluation)
+    rc = launch_jacobian(p, Y, t + dt, J, F, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_comb, dim3(gs), dim3(TB), 0, st, R, Y, cnp1, Yn, cn,
+                       scheme == PDA_BDF2 ? d_Uprev : (scheme == PDA_CRANK_NICOLSON ? Fn : nullptr),
+                       scheme == PDA_BDF2 ? cnm1 : cfn, F, cf, n);
+    PDA_HIP_CHECK(hipGetLastError());
+    S.residual_norm = norm2(*ip, R, st);
+    if (ip->stop == PDA_STOP_RESIDUAL && S.residual_norm <= ip->tol) {
+      S.converged = 1;
+      break;
+    }
+    // A = cnp1 I + cf df/dy; solve A dx = -R
+    hipLaunchKernelGGL(k_discrete_jac, dim3(grid_of(n)), dim3(TB), 0, st, p.d_rowptr.as<int64_t>(),
+                       p.d_colidx.as<int32_t>(), J, n, cf, cnp1);
+    hipLaunchKernelGGL(k_comb, dim3(gs), dim3(TB), 0, st, R, R, -1., nullptr, 0., nullptr, 0., nullptr, 0., n);
+    PDA_HIP_CHECK(hipGetLastError());
+    int li = 0;
+    double lr = 0.;
+    rc = bicgstab(*ip, ip->R.as<double>(), dx, st, &li, &lr);
+    if (rc) return rc;
+    S.linear_iters += li;
+    S.last_linear_rel_residual = lr;
+    S.newton_iters = it + 1;
+    S.correction_norm = norm2(*ip, dx, st);
+    if (!std::isfinite(S.correction_norm)) {
+      set_error("pda_implicit_step: Newton correction is not finite");
+      return PDA_E_STATE;
+    }
+    if (ip->stop == PDA_STOP_CORRECTION && S.correction_norm < ip->tol) {
+      S.converged = 1;
+      break;
+    }
+    hipLaunchKernelGGL(k_comb, dim3(gs), dim3(TB), 0, st, Y, Y, 1., dx, 1., nullptr, 0., nullptr, 0., n);
+    PDA_HIP_CHECK(hipGetLastError());
+  }
+  PDA_HIP_CHECK(hipMemcpyAsync(d_U, Y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  PDA_HIP_CHECK(hipStreamSynchronize(st));
+  ip->last = S;
+  if (stats) *stats = S;
+  if (!S.converged) {
+    set_error("pda_implicit_step: Newton did not converge within the iteration limit");
+    return PDA_E_STATE;
+  }
+  return PDA_OK;
+}
+
+}  // extern "C"

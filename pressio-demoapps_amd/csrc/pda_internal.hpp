@@ -172,6 +172,8 @@ struct pda_problem_s {
 
 namespace pda {
 int ensure_device(pda_problem_s &p);
+// the scalar CSR pattern (d_rowptr, d_colidx) on the device
+int ensure_pattern_device(pda_problem_s &p);
 int launch_velocity(const pda_problem_s &p, const double *U, double t, double *V, hipStream_t s);
 int launch_jacobian(const pda_problem_s &p, const double *U, double t, double *J,
                     double *V, hipStream_t s);

@@ -424,6 +424,9 @@ int ensure_scalar_pattern(pda_problem_s &p) {
 }  // namespace
 
 namespace pda {
+// scalar CSR pattern on the host and the device (made on first use)
+int ensure_pattern_device(pda_problem_s &p) { return ensure_scalar_pattern(p); }
+
 // device copies of the index maps, made on the first evaluation
 int ensure_device(pda_problem_s &p) {
   if (p.on_device) return PDA_OK;

@@ -28,6 +28,17 @@ def library_path():
     return _LIBPATH
 
 
+class ImplicitStats(ctypes.Structure):
+    """pda_implicit_stats"""
+    _fields_ = [("newton_iters", ctypes.c_int), ("linear_iters", ctypes.c_int),
+                ("converged", ctypes.c_int), ("residual_norm", ctypes.c_double),
+                ("correction_norm", ctypes.c_double),
+                ("last_linear_rel_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 def lib():
     """Load libpda.so (raises if it was not built)."""
     global _lib
@@ -65,6 +76,10 @@ def lib():
             "pda_stepper_create": ([vp, i32, P(vp)], i32),
             "pda_stepper_advance": ([vp, vp, dbl, dbl, i64, i32, vp], i32),
             "pda_stepper_destroy": ([vp], None),
+            "pda_implicit_create": ([vp, i32, P(vp)], i32),
+            "pda_implicit_set_options": ([vp, i32, dbl, i32, dbl, i32], i32),
+            "pda_implicit_step": ([vp, vp, vp, vp, dbl, dbl, vp, P(ImplicitStats)], i32),
+            "pda_implicit_destroy": ([vp], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -495,3 +510,54 @@ def advanceSSP3(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showPro
                 use_graph=True):
     """pressiodemoapps.advanceSSP3 (pressiodemoapps/__init__.py:155-185), on the device."""
     _advance(StepScheme.SSPRK3, appObj, state, dt, Nsteps, startTime, observer, use_graph)
+
+
+# ------------------------------------------------------- implicit stepping
+class ImplicitScheme(enum.IntEnum):
+    BDF1 = 1
+    BDF2 = 2
+    CrankNicolson = 3
+
+
+class StopRule(enum.IntEnum):
+    Correction = 0   # pressio default: ||dx|| < tol, dx not applied
+    Residual = 1     # ||R(y)|| <= tol
+
+
+class ImplicitStepper:
+    """Device implicit stepper (pda_implicit_*): pressio's BDF1 / BDF2 /
+    Crank-Nicolson residual and Jacobian, Newton on the exact Jacobian, a
+    device BiCGSTAB with block-Jacobi preconditioning."""
+
+    def __init__(self, problem, scheme):
+        self._p = problem
+        h = ctypes.c_void_p()
+        _check(lib().pda_implicit_create(problem.handle, int(scheme), ctypes.byref(h)))
+        self._h = h.value
+
+    def set_options(self, stop=StopRule.Correction, tol=1e-6, max_newton=100, lin_rtol=1e-14,
+                    lin_max=0):
+        _check(lib().pda_implicit_set_options(self._h, int(stop), float(tol), int(max_newton),
+                                              float(lin_rtol), int(lin_max)))
+
+    def step(self, state, t, dt, prev=None, guess=None):
+        """y_n (state, in place) -> y_{n+1}; prev = y_{n-1} for BDF2 (None:
+        BDF1 step); guess = Newton's initial iterate (None: y_n).  Returns
+        the step's statistics as a dict."""
+        n = self._p.totalDofStencilMesh()
+        _dev_tensor(state, n, "state")
+        if prev is not None:
+            _dev_tensor(prev, n, "prev")
+        if guess is not None:
+            _dev_tensor(guess, n, "guess")
+        st = ImplicitStats()
+        _check(lib().pda_implicit_step(self._h, _ptr(state), _ptr(prev) if prev is not None else None,
+                                       _ptr(guess) if guess is not None else None, float(t),
+                                       float(dt), _stream(), ctypes.byref(st)))
+        return st.as_dict()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pda_implicit_destroy(h)
+            self._h = None

@@ -54,6 +54,11 @@ TRAJ = [
     "tests_cpp/eigen_2d_burgers_periodic_explicit/weno5/gold.txt",
     "tests_cpp/eigen_3d_euler_sedov_symmetry/firstorder_nxnynz_equal/gold_state.txt",
     "tests_cpp/eigen_3d_euler_sedov_symmetry/weno3_nxnynz_equal/gold_state.txt",
+    # implicit (BDF1 with pressio's Newton + BiCGSTAB; Crank-Nicolson)
+    "tests_cpp/eigen_1d_euler_sod_implicit/firstorder/gold.txt",
+    "tests_cpp/eigen_1d_euler_sod_implicit/weno3/gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_implicit/firstorder/rho_gold.txt",
+    "tests_cpp/eigen_2d_euler_riemann_implicit/firstorder/p_gold.txt",
 ]
 
 

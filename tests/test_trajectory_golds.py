@@ -99,3 +99,74 @@ def test_sedov3d_ssprk3(scheme, st, name):
     U = ssprk3(f, U0, 1e-4, 250)
     gold = np.load(os.path.join(TRAJ, f"eigen_3d_euler_sedov_symmetry__{name}_nxnynz_equal__gold_state.npy"))
     assert np.allclose(U, gold, rtol=1e-8, atol=1e-10)
+
+
+# ------------------------------------------------------------- implicit
+def newton_implicit(scheme, y_n, y_nm1, dt, fj, tol, f_n=None):
+    """One implicit step with pressio's discrete residual/Jacobian
+    (ode_implicit_discrete_residual.hpp:112-240, ode_implicit_discrete_jacobian.hpp:60-121)
+    and its Newton loop: solve, stop when ||dx|| < tol without applying dx
+    (solvers_nonlinear/impl/nonlinear_least_squares.hpp:70-84); the linear
+    solves are direct (pressio's BiCGSTAB runs to machine precision)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    cnp1, cn, cnm1, cf = 1.0, -1.0, 0.0, -dt
+    if scheme == "bdf2":
+        cn, cnm1, cf = -4.0 / 3.0, 1.0 / 3.0, -(2.0 / 3.0) * dt
+    elif scheme == "cn":
+        cf = -0.5 * dt
+    y = y_n.copy()
+    n = len(y)
+    for _ in range(100):
+        F, rp, ci, vals = fj(y)
+        R = cnp1 * y + cn * y_n + cf * F
+        if scheme == "bdf2":
+            R += cnm1 * y_nm1
+        if scheme == "cn":
+            R += -0.5 * dt * f_n
+        A = sp.csr_matrix((cf * vals, ci, rp), shape=(n, n)) + cnp1 * sp.identity(n, format="csr")
+        dx = spla.spsolve(A.tocsc(), -R)
+        if np.linalg.norm(dx) < tol:
+            return y
+        y = y + dx
+    raise AssertionError("Newton did not converge")
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3")])
+def test_sod1d_bdf1_implicit(scheme, st, name):
+    # tests_cpp/eigen_1d_euler_sod_implicit: 100 cells on [-0.5,0.5], BDF1,
+    # dt 1e-3 x 100, Newton tolerance 1e-6, gold compared at rtol 1e-9, atol 1e-11
+    mesh = pda.create_structured_mesh((100,), (-0.5, 0.5), st)
+    prob = pda.create_problem(mesh, pda.Euler1d.Sod, scheme)
+    U = np.zeros(prob.totalDofStencilMesh())
+    assert pda.lib().pda_initial_condition_host(prob.handle, U.ctypes.data_as(ctypes.c_void_p)) == 0
+    orc = pyoracle.Oracle()
+    ms = orc.from_mesh(mesh)
+    pb = orc.problem(1, int(pda.Euler1d.Sod), int(scheme), [1.4])
+    fj = lambda y: orc.velocity_and_jacobian(ms, pb, y)  # noqa: E731
+    for _ in range(100):
+        U = newton_implicit("bdf1", U, None, 1e-3, fj, 1e-6)
+    gold = np.load(os.path.join(TRAJ, f"eigen_1d_euler_sod_implicit__{name}__gold.npy"))
+    assert np.allclose(U, gold, rtol=1e-9, atol=1e-11)
+
+
+def test_riemann2d_cn_implicit():
+    # tests_cpp/eigen_2d_euler_riemann_implicit/firstorder: 20x20, IC2,
+    # Crank-Nicolson dt 0.02 x 30, Newton tolerance 1e-5
+    mesh = pda.create_structured_mesh((20, 20), (0.0, 1.0, 0.0, 1.0), 3)
+    prob = pda.create_problem(mesh, pda.Euler2d.Riemann, S.FirstOrder, 2)
+    U = np.zeros(prob.totalDofStencilMesh())
+    assert pda.lib().pda_initial_condition_host(prob.handle, U.ctypes.data_as(ctypes.c_void_p)) == 0
+    orc = pyoracle.Oracle()
+    ms = orc.from_mesh(mesh)
+    pb = orc.problem(2, int(pda.Euler2d.Riemann), 0, [1.4])
+    fj = lambda y: orc.velocity_and_jacobian(ms, pb, y)  # noqa: E731
+    for _ in range(30):
+        fn = orc.velocity_and_jacobian(ms, pb, U)[0]
+        U = newton_implicit("cn", U, None, 0.02, fj, 1e-5, f_n=fn)
+    U = U.reshape(-1, 4)
+    rho = U[:, 0]
+    p = 0.4 * (U[:, 3] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / rho)
+    base = "eigen_2d_euler_riemann_implicit__firstorder__"
+    assert np.allclose(rho, np.load(os.path.join(TRAJ, base + "rho_gold.npy")), rtol=1e-10, atol=1e-12)
+    assert np.allclose(p, np.load(os.path.join(TRAJ, base + "p_gold.npy")), rtol=1e-10, atol=1e-12)
