@@ -205,6 +205,40 @@ int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev,
                       pda_implicit_stats *stats);
 void pda_implicit_destroy(pda_implicit ip);
 
+/* ------------------------------------------------- learned Newton guess */
+
+/* A 4-layer 3x3 convolutional network (32 channels, BF16 weights and
+   activations, FP32 accumulation on MFMA) predicting the next state of a
+   periodic nx x ny two-component field from (y_n, y_{n-1}, nu):
+     guess = 2 y_n - y_{n-1} + s_o * net(y_n, (y_n - y_{n-1})/s_d, (nu - nu_mean)/nu_scale).
+   No counterpart in the reference: it feeds pda_implicit_step's d_guess,
+   and Newton on the exact Jacobian still decides the converged state.
+   States are device arrays of nx*ny*2 doubles in cell order j*nx + i.
+   nx*ny must be a multiple of 512. */
+typedef struct pda_guessnet_s *pda_guessnet;
+int pda_guessnet_create(int nx, int ny, pda_guessnet *out);
+void pda_guessnet_destroy(pda_guessnet net);
+int pda_guessnet_num_params(pda_guessnet net, int64_t *n);
+/* FP32 master parameters, per layer: W[tap][cout][cin] then bias[cout];
+   layers (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 2).  Setting them
+   resets the optimizer state. */
+int pda_guessnet_set_params(pda_guessnet net, const float *w, int64_t n);
+int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n);
+int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale);
+int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1, double nu,
+                         double *d_guess, pda_stream stream);
+/* loss = mean((net - (y_{n+1} - 2 y_n + y_{n-1})/s_o)^2) and its gradient
+   w.r.t. the parameters (host arrays, nullable; out = the net's raw
+   output, nx*ny*2 floats). */
+int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1,
+                           const double *d_unp1, double nu, float *grad, float *loss,
+                           float *out, pda_stream stream);
+/* one Adam step (beta 0.9 / 0.999) on that loss; loss (nullable) is
+   reported before the update */
+int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1,
+                            const double *d_unp1, double nu, float lr, float *loss,
+                            pda_stream stream);
+
 /* Block view of the Jacobian (m x m blocks).  For each block-row r and block
    slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values
    start at d_Jvals[rowptr[r*m] + s*m] with row stride nblk[r]*m. */

@@ -28,7 +28,8 @@ COMMON = [
 ]
 
 SOURCES = ["pda_mesh.cpp", "pda_problem.cpp", "pda_kernels.hip", "pda_box_kernels.hip",
-           "pda_stepper.hip", "pda_implicit.hip"]
+           "pda_stepper.hip", "pda_implicit.hip",
+           "pda_guessnet.hip"]
 
 
 def _compile(src):

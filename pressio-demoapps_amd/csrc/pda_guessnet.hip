@@ -1,0 +1,565 @@
+// pda_guessnet.hip -- the learned initial guess for implicit Newton steps on
+// periodic 2D two-component fields (the Burgers problem): a 4-layer 3x3
+// convolutional network, 32 channels, BF16 weights and activations with
+// FP32 accumulation on the MFMA units (v_mfma_f32_16x16x32_bf16), trained
+// on the device by the kernels below (forward, backward-data,
+// backward-weight, Adam).
+//
+//   features (8 ch): u_n, v_n, (u_n - u_{n-1})/s_d, (v_n - v_{n-1})/s_d, nu_hat, 0, 0, 0
+//   L1 8->32 SiLU, L2 32->32 SiLU, L3 32->32 SiLU, L4 32->2 (linear)
+//   guess  = 2 y_n - y_{n-1} + s_o * L4          (periodic padding)
+//
+// Every convolution is an implicit GEMM: D[pixel][cout] += A[pixel][k] B[k][cout]
+// over K = taps x input channels, 32 K per MFMA.  An MFMA pairs A's and B's K
+// elements by register position, so any consistent assignment of K to
+// (lane group, register element) is valid; the kernels use K = 8*g + j for
+// lane group g = lane/16 and element j.  The output layout used is the
+// 16x16 MFMA's: lane l holds D[4*(l/16) + i][l%16], i = 0..3; A row = l%16,
+// B column = l%16.
+#include <cmath>
+
+#include "pda_internal.hpp"
+
+namespace pda {
+namespace nn {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int C = 32;      // hidden channels
+constexpr int CIN0 = 8;    // feature channels (5 used)
+constexpr int NOUT = 2;    // output channels
+constexpr int NL = 4;      // layers
+constexpr int MT = 4;      // 16-pixel M tiles per wavefront in the forward kernels
+
+// layer geometry: input channels, output channels
+__host__ __device__ constexpr int cin_of(int l) { return l == 0 ? CIN0 : C; }
+__host__ __device__ constexpr int cout_of(int l) { return l == NL - 1 ? NOUT : C; }
+// K steps of 32 of a forward layer: 9 taps x 32 channels, or 9 taps x 8
+// channels packed four taps per step
+__host__ __device__ constexpr int ksteps(int cin) { return cin == C ? 9 : 3; }
+
+__device__ __forceinline__ float silu(float z) { return z / (1.f + __expf(-z)); }
+__device__ __forceinline__ float dsilu(float z) {
+  const float s = 1.f / (1.f + __expf(-z));
+  return s * (1.f + z * (1.f - s));
+}
+
+__device__ __forceinline__ int64_t shifted(int64_t p, int tap, int H, int W) {
+  const int y = int(p / W), x = int(p - int64_t(y) * W);
+  int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+  yy = yy < 0 ? yy + H : (yy >= H ? yy - H : yy);
+  xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+  return int64_t(yy) * W + xx;
+}
+
+// A operand: 8 bf16 of pixel p for K step s, lane group g
+template <int CIN>
+__device__ __forceinline__ bf16x8 load_a(const __bf16 *__restrict__ X, int64_t p, int s, int g, int H, int W) {
+  if constexpr (CIN == C) {
+    return *reinterpret_cast<const bf16x8 *>(X + shifted(p, s, H, W) * C + 8 * g);
+  } else {
+    const int tap = 4 * s + g;
+    if (tap >= 9) {
+      bf16x8 z;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) z[j] = __bf16(0.f);
+      return z;
+    }
+    return *reinterpret_cast<const bf16x8 *>(X + shifted(p, tap, H, W) * CIN0);
+  }
+}
+
+// Forward convolution (and, with MODE_BWD, the backward-data convolution).
+//   Wp: packed weights [kstep][ncout_pad][32] bf16 (B operand rows)
+//   MODE_HIDDEN: Z = acc + bias (FP32, kept for the backward), Y = SiLU(Z) (bf16)
+//   MODE_OUT:    out[p][c] = acc + bias for c < NOUT (FP32)
+//   MODE_BWD:    dZprev[p][c] = acc * SiLU'(Zprev[p][c]) (bf16), no bias
+enum { MODE_HIDDEN = 0, MODE_OUT = 1, MODE_BWD = 2 };
+
+template <int CIN, int NT, int MODE>
+__global__ void __launch_bounds__(64) k_conv(const __bf16 *__restrict__ X, const __bf16 *__restrict__ Wp,
+                                              const float *__restrict__ bias, int H, int W,
+                                              float *__restrict__ Z, __bf16 *__restrict__ Y,
+                                              float *__restrict__ out, const float *__restrict__ Zprev) {
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  const int64_t p0 = int64_t(blockIdx.x) * (16 * MT);
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+  constexpr int KS = ksteps(CIN);
+  for (int s = 0; s < KS; ++s) {
+    bf16x8 b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+      b[nt] = *reinterpret_cast<const bf16x8 *>(Wp + (int64_t(s) * (16 * NT) + 16 * nt + r) * 32 + 8 * g);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const bf16x8 a = load_a<CIN>(X, p0 + 16 * mt + r, s, g, H, W);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt)
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[mt][nt], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int64_t p = p0 + 16 * mt + 4 * g + i;
+        const int c = 16 * nt + r;
+        if constexpr (MODE == MODE_HIDDEN) {
+          const float z = acc[mt][nt][i] + bias[c];
+          Z[p * C + c] = z;
+          Y[p * C + c] = __bf16(silu(z));
+        } else if constexpr (MODE == MODE_OUT) {
+          if (c < NOUT) out[p * NOUT + c] = acc[mt][nt][i] + bias[c];
+        } else {
+          Y[p * C + c] = __bf16(acc[mt][nt][i] * dsilu(Zprev[p * C + c]));
+        }
+      }
+}
+
+// Backward-weight: partial dW[tap][cout][cin] over a chunk of 32*KCH pixels,
+// as MFMAs with M = cout, N = cin, K = pixels.  dZ (bf16 [p][32], zero
+// beyond the layer's outputs) and X (bf16 [p][CIN]) tiles of 32 pixels are
+// staged in LDS so that each lane gathers 8 consecutive pixels of one
+// channel.
+constexpr int KCH = 16; // 32-pixel K steps per chunk
+
+template <int CIN>
+__global__ void __launch_bounds__(64) k_conv_wgrad(const __bf16 *__restrict__ dZ, const __bf16 *__restrict__ X,
+                                                    int H, int W, int coutp, float *__restrict__ part) {
+  constexpr int NTN = CIN == C ? 2 : 1; // 16-wide cin tiles
+  __shared__ __bf16 sz[32][C + 2];
+  __shared__ __bf16 sx[32][CIN + 2];
+  const int lane = threadIdx.x, r = lane & 15, g = lane >> 4;
+  const int tap = blockIdx.y, chunk = blockIdx.x;
+  const int MTN = coutp / 16; // 16-wide cout tiles
+  f32x4 acc[2][NTN];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < NTN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int ks = 0; ks < KCH; ++ks) {
+    const int64_t pb = (int64_t(chunk) * KCH + ks) * 32;
+    // stage dZ[pb .. pb+31][0..31] and X[shift(pb + i, tap)][0..CIN-1]
+    for (int q = lane; q < 32 * C; q += 64) sz[q / C][q % C] = dZ[(pb + q / C) * C + q % C];
+    for (int q = lane; q < 32 * CIN; q += 64)
+      sx[q / CIN][q % CIN] = X[shifted(pb + q / CIN, tap, H, W) * CIN + q % CIN];
+    __syncthreads();
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      if (mt >= MTN) break;
+      bf16x8 a;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = sz[8 * g + j][16 * mt + r];
+#pragma unroll
+      for (int nt = 0; nt < NTN; ++nt) {
+        bf16x8 b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int cin = 16 * nt + r;
+          b[j] = cin < CIN ? sx[8 * g + j][cin < CIN ? cin : 0] : __bf16(0.f);
+        }
+        acc[mt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[mt][nt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // part[chunk][tap][cout][cin]
+  float *dst = part + (int64_t(chunk) * 9 + tap) * (C * C);
+#pragma unroll
+  for (int mt = 0; mt < 2; ++mt) {
+    if (mt >= MTN) break;
+#pragma unroll
+    for (int nt = 0; nt < NTN; ++nt)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int cout = 16 * mt + 4 * g + i, cin = 16 * nt + r;
+        if (cin < CIN) dst[cout * C + cin] = acc[mt][nt][i];
+      }
+  }
+}
+
+// sum of the partials over chunks -> dW[tap][cout][cin] (FP32); bias
+// gradient db[cout] = sum_p dZ[p][cout]
+__global__ void k_wgrad_reduce(const float *__restrict__ part, int nchunk, int cin, int cout,
+                               float *__restrict__ dW) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x; // tap*cout*cin
+  if (e >= 9 * cout * cin) return;
+  const int tap = e / (cout * cin), co = (e / cin) % cout, ci = e % cin;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; ++c) s += part[(int64_t(c) * 9 + tap) * (C * C) + co * C + ci];
+  dW[e] = s;
+}
+
+__global__ void k_bias_grad(const __bf16 *__restrict__ dZ, int64_t npix, int cout, float *__restrict__ db) {
+  __shared__ float sh[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (int64_t p = threadIdx.x; p < npix; p += 256) s += float(dZ[p * C + c]);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[c] = sh[0];
+}
+
+// features of the network input
+__global__ void k_features(const double *__restrict__ un, const double *__restrict__ unm1, double nu_hat,
+                           double inv_sd, int64_t npix, __bf16 *__restrict__ X0) {
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= npix) return;
+  const double u = un[2 * p], v = un[2 * p + 1];
+  const double du = (u - unm1[2 * p]) * inv_sd, dv = (v - unm1[2 * p + 1]) * inv_sd;
+  __bf16 *x = X0 + p * CIN0;
+  x[0] = __bf16(float(u));
+  x[1] = __bf16(float(v));
+  x[2] = __bf16(float(du));
+  x[3] = __bf16(float(dv));
+  x[4] = __bf16(float(nu_hat));
+  x[5] = x[6] = x[7] = __bf16(0.f);
+}
+
+// guess = 2 y_n - y_{n-1} + s_o * out
+__global__ void k_guess(const double *__restrict__ un, const double *__restrict__ unm1,
+                        const float *__restrict__ out, double so, int64_t n, double *__restrict__ guess) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  guess[i] = 2. * un[i] - unm1[i] + so * double(out[i]);
+}
+
+// loss = mean((out - t)^2), t = (y_{n+1} - 2 y_n + y_{n-1}) / s_o; dZ of the
+// output layer (bf16, 32 channels, zero beyond NOUT); per-block loss sums
+__global__ void k_loss(const float *__restrict__ out, const double *__restrict__ unp1,
+                       const double *__restrict__ un, const double *__restrict__ unm1, double inv_so,
+                       int64_t npix, __bf16 *__restrict__ dZ, float *__restrict__ lpart) {
+  __shared__ float sh[256];
+  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (p < npix) {
+    const float scale = 2.f / float(npix * NOUT);
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+      const float t = float((unp1[2 * p + c] - 2. * un[2 * p + c] + unm1[2 * p + c]) * inv_so);
+      const float d = out[p * NOUT + c] - t;
+      l += d * d;
+      dZ[p * C + c] = __bf16(scale * d);
+    }
+    for (int c = NOUT; c < C; ++c) dZ[p * C + c] = __bf16(0.f);
+  }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
+}
+
+// Adam on the FP32 master parameters
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ gr, float *__restrict__ m1,
+                       float *__restrict__ m2, int64_t n, float lr, float b1, float b2, float eps,
+                       float c1, float c2) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = gr[i];
+  m1[i] = b1 * m1[i] + (1.f - b1) * gi;
+  m2[i] = b2 * m2[i] + (1.f - b2) * gi * gi;
+  w[i] -= lr * (m1[i] * c1) / (sqrtf(m2[i] * c2) + eps);
+}
+
+// Pack the FP32 master parameters of every layer into the bf16 B-operand
+// layouts: forward Wf[l][s][coutp][32] and backward-data Wb[l][tap'][cin][32]
+// (= W[8 - tap'][cout][cin], K = cout).
+__global__ void k_pack(const float *__restrict__ w, const int64_t *__restrict__ woff, __bf16 *__restrict__ Wf,
+                       const int64_t *__restrict__ foff, __bf16 *__restrict__ Wb,
+                       const int64_t *__restrict__ boff) {
+  const int l = blockIdx.y;
+  const int cin = cin_of(l), cout = cout_of(l), coutp = cout == NOUT ? 16 : C;
+  const int KS = ksteps(cin);
+  const float *W = w + woff[l]; // [tap][cout][cin]
+  const int nf = KS * coutp * 32;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nf; e += gridDim.x * blockDim.x) {
+    const int s = e / (coutp * 32), co = (e / 32) % coutp, k = e % 32;
+    float v = 0.f;
+    if (co < cout) {
+      if (cin == C) {
+        v = W[(s * cout + co) * cin + k];
+      } else {
+        const int tap = 4 * s + k / 8, ci = k % 8;
+        v = tap < 9 ? W[(tap * cout + co) * cin + ci] : 0.f;
+      }
+    }
+    Wf[foff[l] + e] = __bf16(v);
+  }
+  if (l == 0) return; // no input gradient for the first layer
+  const int nb = 9 * C * 32;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nb; e += gridDim.x * blockDim.x) {
+    const int tp = e / (C * 32), ci = (e / 32) % C, k = e % 32;
+    const float v = k < cout ? W[((8 - tp) * cout + k) * cin + ci] : 0.f;
+    Wb[boff[l] + e] = __bf16(v);
+  }
+}
+
+}  // namespace nn
+}  // namespace pda
+
+struct pda_guessnet_s {
+  int H = 0, W = 0;
+  int64_t npix = 0;
+  double s_d = 1e-2, s_o = 1e-4, nu_mean = 5.5e-3, nu_scale = 4.5e-3;
+  int64_t nparam = 0;
+  int64_t woff[pda::nn::NL + 1] = {0};
+  int64_t foff[pda::nn::NL + 1] = {0}, boff[pda::nn::NL + 1] = {0};
+  pda::DevBuf w, g, m1, m2;          // FP32 master params, grads, Adam moments
+  pda::DevBuf Wf, Wb, d_woff, d_foff, d_boff;
+  pda::DevBuf X0, Y[3], Z[3], dZa, dZb, out, part, lpart;
+  int adam_t = 0;
+};
+
+namespace {
+using namespace pda;
+using namespace pda::nn;
+
+int pack(pda_guessnet_s &net, hipStream_t st) {
+  hipLaunchKernelGGL(k_pack, dim3(16, NL), dim3(256), 0, st, net.w.as<float>(), net.d_woff.as<int64_t>(),
+                     net.Wf.as<__bf16>(), net.d_foff.as<int64_t>(), net.Wb.as<__bf16>(),
+                     net.d_boff.as<int64_t>());
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+const float *bias_of(const pda_guessnet_s &net, int l) {
+  return net.w.as<float>() + net.woff[l] + 9 * cin_of(l) * cout_of(l);
+}
+
+int features(pda_guessnet_s &net, const double *un, const double *unm1, double nu, hipStream_t st) {
+  hipLaunchKernelGGL(k_features, dim3(unsigned((net.npix + 255) / 256)), dim3(256), 0, st, un, unm1,
+                     (nu - net.nu_mean) / net.nu_scale, 1. / net.s_d, net.npix, net.X0.as<__bf16>());
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+// forward through all layers from the features in X0; out = L4 (FP32 [p][2])
+int forward(pda_guessnet_s &net, hipStream_t st) {
+  const unsigned grid = unsigned(net.npix / (16 * MT));
+  const __bf16 *Wf = net.Wf.as<__bf16>();
+  hipLaunchKernelGGL((k_conv<CIN0, 2, MODE_HIDDEN>), dim3(grid), dim3(64), 0, st, net.X0.as<__bf16>(),
+                     Wf + net.foff[0], bias_of(net, 0), net.H, net.W, net.Z[0].as<float>(),
+                     net.Y[0].as<__bf16>(), nullptr, nullptr);
+  for (int l = 1; l < NL - 1; ++l)
+    hipLaunchKernelGGL((k_conv<C, 2, MODE_HIDDEN>), dim3(grid), dim3(64), 0, st, net.Y[l - 1].as<__bf16>(),
+                       Wf + net.foff[l], bias_of(net, l), net.H, net.W, net.Z[l].as<float>(),
+                       net.Y[l].as<__bf16>(), nullptr, nullptr);
+  hipLaunchKernelGGL((k_conv<C, 1, MODE_OUT>), dim3(grid), dim3(64), 0, st, net.Y[NL - 2].as<__bf16>(),
+                     Wf + net.foff[NL - 1], bias_of(net, NL - 1), net.H, net.W, nullptr, nullptr,
+                     net.out.as<float>(), nullptr);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+// Gradients (into net.g) of the loss whose output-layer dZ is in dZa.
+// Layer l: dW_l from (dZ_l, input_l); dZ_{l-1} = (dZ_l conv flipped
+// transposed W_l) * SiLU'(Z_{l-1}), alternating between dZa and dZb.
+int backward(pda_guessnet_s &net, hipStream_t st) {
+  const unsigned grid = unsigned(net.npix / (16 * MT));
+  const int nchunk = int(net.npix / (32 * KCH));
+  float *gW = net.g.as<float>();
+  const __bf16 *Wb = net.Wb.as<__bf16>();
+  __bf16 *dz = net.dZa.as<__bf16>(), *dz_next = net.dZb.as<__bf16>();
+  for (int l = NL - 1; l >= 0; --l) {
+    const int cin = cin_of(l), cout = cout_of(l), coutp = cout == NOUT ? 16 : C;
+    const __bf16 *Xin = l == 0 ? net.X0.as<__bf16>() : net.Y[l - 1].as<__bf16>();
+    if (cin == C)
+      hipLaunchKernelGGL((k_conv_wgrad<C>), dim3(nchunk, 9), dim3(64), 0, st, dz, Xin, net.H, net.W, coutp,
+                         net.part.as<float>());
+    else
+      hipLaunchKernelGGL((k_conv_wgrad<CIN0>), dim3(nchunk, 9), dim3(64), 0, st, dz, Xin, net.H, net.W,
+                         coutp, net.part.as<float>());
+    const int ne = 9 * cout * cin;
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((ne + 255) / 256), dim3(256), 0, st, net.part.as<float>(), nchunk,
+                       cin, cout, gW + net.woff[l]);
+    hipLaunchKernelGGL(k_bias_grad, dim3(cout), dim3(256), 0, st, dz, net.npix, cout,
+                       gW + net.woff[l] + 9 * cin * cout);
+    PDA_HIP_CHECK(hipGetLastError());
+    if (l == 0) break;
+    hipLaunchKernelGGL((k_conv<C, 2, MODE_BWD>), dim3(grid), dim3(64), 0, st, dz, Wb + net.boff[l], nullptr,
+                       net.H, net.W, nullptr, dz_next, nullptr, net.Z[l - 1].as<float>());
+    PDA_HIP_CHECK(hipGetLastError());
+    std::swap(dz, dz_next);
+  }
+  return PDA_OK;
+}
+
+// loss of the current forward against y_{n+1}; leaves the output dZ in dZa
+int loss(pda_guessnet_s &net, const double *un, const double *unm1, const double *unp1, float *lval,
+         hipStream_t st) {
+  const unsigned nb = unsigned((net.npix + 255) / 256);
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, net.out.as<float>(), unp1, un, unm1, 1. / net.s_o,
+                     net.npix, net.dZa.as<__bf16>(), net.lpart.as<float>());
+  PDA_HIP_CHECK(hipGetLastError());
+  if (lval) {
+    std::vector<float> h(nb);
+    PDA_HIP_CHECK(hipMemcpyAsync(h.data(), net.lpart.ptr, sizeof(float) * nb, hipMemcpyDeviceToHost, st));
+    PDA_HIP_CHECK(hipStreamSynchronize(st));
+    double sum = 0.;
+    for (float v : h) sum += v;
+    *lval = float(sum / double(net.npix * NOUT));
+  }
+  return PDA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pda_guessnet_create(int nx, int ny, pda_guessnet *out) {
+  if (!out || nx < 4 || ny < 4 || (int64_t(nx) * ny) % (32 * KCH) != 0) {
+    set_error("pda_guessnet_create: need nx, ny >= 4 and nx*ny a multiple of 512");
+    return PDA_E_INVALID_ARG;
+  }
+  auto *net = new pda_guessnet_s();
+  net->W = nx;
+  net->H = ny;
+  net->npix = int64_t(nx) * ny;
+  int64_t wo = 0, fo = 0, bo = 0;
+  for (int l = 0; l < NL; ++l) {
+    net->woff[l] = wo;
+    net->foff[l] = fo;
+    net->boff[l] = bo;
+    const int cin = cin_of(l), cout = cout_of(l), coutp = cout == NOUT ? 16 : C;
+    wo += 9 * cin * cout + cout;
+    fo += int64_t(ksteps(cin)) * coutp * 32;
+    bo += l == 0 ? 0 : 9 * C * 32;
+  }
+  net->woff[NL] = wo;
+  net->foff[NL] = fo;
+  net->boff[NL] = bo;
+  net->nparam = wo;
+  const int64_t np = net->npix;
+  hipError_t e = net->w.alloc(sizeof(float) * wo);
+  if (e == hipSuccess) e = net->g.alloc(sizeof(float) * wo);
+  if (e == hipSuccess) e = net->m1.alloc(sizeof(float) * wo);
+  if (e == hipSuccess) e = net->m2.alloc(sizeof(float) * wo);
+  if (e == hipSuccess) e = net->Wf.alloc(sizeof(__bf16) * fo);
+  if (e == hipSuccess) e = net->Wb.alloc(sizeof(__bf16) * std::max<int64_t>(bo, 1));
+  if (e == hipSuccess) e = net->d_woff.upload(std::vector<int64_t>(net->woff, net->woff + NL + 1));
+  if (e == hipSuccess) e = net->d_foff.upload(std::vector<int64_t>(net->foff, net->foff + NL + 1));
+  if (e == hipSuccess) e = net->d_boff.upload(std::vector<int64_t>(net->boff, net->boff + NL + 1));
+  if (e == hipSuccess) e = net->X0.alloc(sizeof(__bf16) * np * CIN0);
+  for (int i = 0; i < 3 && e == hipSuccess; ++i) e = net->Y[i].alloc(sizeof(__bf16) * np * C);
+  for (int i = 0; i < 3 && e == hipSuccess; ++i) e = net->Z[i].alloc(sizeof(float) * np * C);
+  if (e == hipSuccess) e = net->dZa.alloc(sizeof(__bf16) * np * C);
+  if (e == hipSuccess) e = net->dZb.alloc(sizeof(__bf16) * np * C);
+  if (e == hipSuccess) e = net->out.alloc(sizeof(float) * np * NOUT);
+  if (e == hipSuccess) e = net->part.alloc(sizeof(float) * (np / (32 * KCH)) * 9 * C * C);
+  if (e == hipSuccess) e = net->lpart.alloc(sizeof(float) * ((np + 255) / 256));
+  if (e == hipSuccess) e = hipMemset(net->w.ptr, 0, sizeof(float) * wo);
+  if (e == hipSuccess) e = hipMemset(net->m1.ptr, 0, sizeof(float) * wo);
+  if (e == hipSuccess) e = hipMemset(net->m2.ptr, 0, sizeof(float) * wo);
+  if (e != hipSuccess) {
+    set_error(std::string("pda_guessnet_create: ") + hipGetErrorString(e));
+    delete net;
+    return PDA_E_HIP;
+  }
+  *out = net;
+  return PDA_OK;
+}
+
+void pda_guessnet_destroy(pda_guessnet net) { delete net; }
+
+int pda_guessnet_num_params(pda_guessnet net, int64_t *n) {
+  if (!net || !n) return PDA_E_INVALID_ARG;
+  *n = net->nparam;
+  return PDA_OK;
+}
+
+int pda_guessnet_set_params(pda_guessnet net, const float *w, int64_t n) {
+  if (!net || !w || n != net->nparam) {
+    set_error("pda_guessnet_set_params: expected the network's parameter count");
+    return PDA_E_INVALID_ARG;
+  }
+  PDA_HIP_CHECK(hipMemcpy(net->w.ptr, w, sizeof(float) * n, hipMemcpyHostToDevice));
+  PDA_HIP_CHECK(hipMemset(net->m1.ptr, 0, sizeof(float) * n));
+  PDA_HIP_CHECK(hipMemset(net->m2.ptr, 0, sizeof(float) * n));
+  net->adam_t = 0;
+  const int rc = pack(*net, nullptr);
+  if (rc) return rc;
+  PDA_HIP_CHECK(hipDeviceSynchronize());
+  return PDA_OK;
+}
+
+int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n) {
+  if (!net || !w || n != net->nparam) return PDA_E_INVALID_ARG;
+  PDA_HIP_CHECK(hipMemcpy(w, net->w.ptr, sizeof(float) * n, hipMemcpyDeviceToHost));
+  return PDA_OK;
+}
+
+int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale) {
+  if (!net || !(s_d > 0.) || !(s_o > 0.) || !(nu_scale > 0.)) return PDA_E_INVALID_ARG;
+  net->s_d = s_d;
+  net->s_o = s_o;
+  net->nu_mean = nu_mean;
+  net->nu_scale = nu_scale;
+  return PDA_OK;
+}
+
+int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1, double nu,
+                         double *d_guess, pda_stream stream) {
+  if (!net || !d_un || !d_unm1 || !d_guess) return PDA_E_INVALID_ARG;
+  hipStream_t st = hipStream_t(stream);
+  int rc = features(*net, d_un, d_unm1, nu, st);
+  if (!rc) rc = forward(*net, st);
+  if (rc) return rc;
+  const int64_t n = net->npix * NOUT;
+  hipLaunchKernelGGL(k_guess, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, d_un, d_unm1,
+                     net->out.as<float>(), net->s_o, n, d_guess);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_unp1,
+                           double nu, float *grad, float *lossv, float *outv, pda_stream stream) {
+  if (!net || !d_un || !d_unm1 || !d_unp1) return PDA_E_INVALID_ARG;
+  hipStream_t st = hipStream_t(stream);
+  int rc = features(*net, d_un, d_unm1, nu, st);
+  if (!rc) rc = forward(*net, st);
+  if (!rc) rc = loss(*net, d_un, d_unm1, d_unp1, lossv, st);
+  if (!rc) rc = backward(*net, st);
+  if (rc) return rc;
+  if (grad)
+    PDA_HIP_CHECK(hipMemcpyAsync(grad, net->g.ptr, sizeof(float) * net->nparam, hipMemcpyDeviceToHost, st));
+  if (outv)
+    PDA_HIP_CHECK(hipMemcpyAsync(outv, net->out.ptr, sizeof(float) * net->npix * NOUT, hipMemcpyDeviceToHost, st));
+  PDA_HIP_CHECK(hipStreamSynchronize(st));
+  return PDA_OK;
+}
+
+int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1,
+                            const double *d_unp1, double nu, float lr, float *lossv, pda_stream stream) {
+  if (!net || !d_un || !d_unm1 || !d_unp1 || !(lr > 0.f)) return PDA_E_INVALID_ARG;
+  hipStream_t st = hipStream_t(stream);
+  int rc = features(*net, d_un, d_unm1, nu, st);
+  if (!rc) rc = forward(*net, st);
+  if (!rc) rc = loss(*net, d_un, d_unm1, d_unp1, lossv, st);
+  if (!rc) rc = backward(*net, st);
+  if (rc) return rc;
+  const float b1 = 0.9f, b2 = 0.999f;
+  net->adam_t += 1;
+  const float c1 = 1.f / (1.f - std::pow(b1, float(net->adam_t)));
+  const float c2 = 1.f / (1.f - std::pow(b2, float(net->adam_t)));
+  hipLaunchKernelGGL(k_adam, dim3(unsigned((net->nparam + 255) / 256)), dim3(256), 0, st, net->w.as<float>(),
+                     net->g.as<float>(), net->m1.as<float>(), net->m2.as<float>(), net->nparam, lr, b1, b2,
+                     1e-8f, c1, c2);
+  PDA_HIP_CHECK(hipGetLastError());
+  return pack(*net, st);
+}
+
+}  // extern "C"

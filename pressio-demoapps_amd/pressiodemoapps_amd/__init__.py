@@ -80,6 +80,16 @@ def lib():
             "pda_implicit_set_options": ([vp, i32, dbl, i32, dbl, i32], i32),
             "pda_implicit_step": ([vp, vp, vp, vp, dbl, dbl, vp, P(ImplicitStats)], i32),
             "pda_implicit_destroy": ([vp], None),
+            "pda_guessnet_create": ([i32, i32, P(vp)], i32),
+            "pda_guessnet_destroy": ([vp], None),
+            "pda_guessnet_num_params": ([vp, P(i64)], i32),
+            "pda_guessnet_set_params": ([vp, P(ctypes.c_float), i64], i32),
+            "pda_guessnet_get_params": ([vp, P(ctypes.c_float), i64], i32),
+            "pda_guessnet_set_scales": ([vp, dbl, dbl, dbl, dbl], i32),
+            "pda_guessnet_predict": ([vp, vp, vp, dbl, vp, vp], i32),
+            "pda_guessnet_gradients": ([vp, vp, vp, vp, dbl, P(ctypes.c_float), P(ctypes.c_float),
+                                        P(ctypes.c_float), vp], i32),
+            "pda_guessnet_train_step": ([vp, vp, vp, vp, dbl, ctypes.c_float, P(ctypes.c_float), vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -560,4 +570,66 @@ class ImplicitStepper:
         h = getattr(self, "_h", None)
         if h and _lib is not None:
             _lib.pda_implicit_destroy(h)
+            self._h = None
+
+
+# ------------------------------------------------------ learned Newton guess
+class GuessNet:
+    """Learned initial guess for implicit steps of a periodic nx x ny
+    two-component field (pda_guessnet_*): guess = 2 y_n - y_{n-1} +
+    s_o * net(y_n, (y_n - y_{n-1})/s_d, nu_hat), a 4-layer 3x3 conv net with
+    BF16 weights and FP32 accumulation on MFMA, trained on the device."""
+
+    def __init__(self, nx, ny):
+        h = ctypes.c_void_p()
+        _check(lib().pda_guessnet_create(int(nx), int(ny), ctypes.byref(h)))
+        self._h = h.value
+        self.nx, self.ny = int(nx), int(ny)
+        n = ctypes.c_int64()
+        _check(lib().pda_guessnet_num_params(self._h, ctypes.byref(n)))
+        self.nparams = n.value
+        self.scales = (1e-2, 1e-4, 5.5e-3, 4.5e-3)
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        _check(lib().pda_guessnet_set_params(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                             len(w)))
+
+    def get_params(self):
+        w = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().pda_guessnet_get_params(self._h, w.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                             len(w)))
+        return w
+
+    def set_scales(self, s_d, s_o, nu_mean, nu_scale):
+        self.scales = (float(s_d), float(s_o), float(nu_mean), float(nu_scale))
+        _check(lib().pda_guessnet_set_scales(self._h, *self.scales))
+
+    def predict(self, un, unm1, nu, out=None):
+        if out is None:
+            out = _torch().empty_like(un)
+        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), float(nu), _ptr(out), _stream()))
+        return out
+
+    def gradients(self, un, unm1, unp1, nu):
+        g = np.zeros(self.nparams, dtype=np.float32)
+        o = np.zeros(self.nx * self.ny * 2, dtype=np.float32)
+        loss = ctypes.c_float()
+        F = ctypes.POINTER(ctypes.c_float)
+        _check(lib().pda_guessnet_gradients(self._h, _ptr(un), _ptr(unm1), _ptr(unp1), float(nu),
+                                            g.ctypes.data_as(F), ctypes.byref(loss), o.ctypes.data_as(F),
+                                            _stream()))
+        return loss.value, g, o
+
+    def train_step(self, un, unm1, unp1, nu, lr, want_loss=False):
+        loss = ctypes.c_float()
+        _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), _ptr(unp1), float(nu),
+                                             float(lr), ctypes.byref(loss) if want_loss else None,
+                                             _stream()))
+        return loss.value if want_loss else None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pda_guessnet_destroy(h)
             self._h = None
