@@ -164,6 +164,32 @@ int pda_stepper_advance(pda_stepper st, double *d_U, double t0, double dt,
                         int64_t nsteps, int use_graph, pda_stream stream);
 void pda_stepper_destroy(pda_stepper st);
 
+/* ------------------------------------------------------------ batching */
+
+/* Many independent instances (no counterpart in the reference, which runs
+   one problem per object): 2D structured problems of one family and scheme
+   whose boundaries are periodic or mirror ghosts (so the velocity kernel
+   covers the whole grid), each with its own grid, parameters and state
+   slice.  The batch state is the concatenation of the instances' states in
+   instance order (offsets from pda_batch_sizes); one launch evaluates every
+   instance's velocity.  A batch must not outlive its problems. */
+typedef struct pda_batch_s *pda_batch;
+int pda_batch_create(const pda_problem *probs, int count, pda_batch *out);
+void pda_batch_destroy(pda_batch b);
+/* total state doubles and, if offsets is non-NULL, count+1 offsets */
+int pda_batch_sizes(pda_batch b, int64_t *total, int64_t *offsets);
+int pda_batch_initial_condition(pda_batch b, double *d_U, pda_stream stream);
+int pda_batch_velocity(pda_batch b, const double *d_U, double t, double *d_V, pda_stream stream);
+/* times the candidate strip heights (rows per workgroup) of the batched
+   velocity kernel on d_U and keeps the fastest; synchronizes the stream */
+int pda_batch_autotune(pda_batch b, const double *d_U, pda_stream stream, int *rows_per_strip,
+                       double *us);
+/* nsteps explicit steps (PDA_FORWARD_EULER / SSPRK3 / RK4 / RK2) of the whole
+   batch; steps_per_graph > 0 records that many steps as one hipGraph
+   (re-recorded when d_U, dt, the scheme or the count change) and replays it */
+int pda_batch_advance(pda_batch b, int scheme, double *d_U, double t0, double dt, int64_t nsteps,
+                      int steps_per_graph, pda_stream stream);
+
 /* ---------------------------------------------------- implicit stepping */
 
 /* Implicit time stepping y_n -> y_{n+1} with pressio's discrete residuals
@@ -188,6 +214,7 @@ typedef struct {
   double residual_norm;            /* ||R||_2 at the last evaluation */
   double correction_norm;          /* ||dx||_2 of the last solve */
   double last_linear_rel_residual; /* ||b - A dx|| / ||b|| of the last solve */
+  double initial_residual_norm;    /* ||R||_2 at the initial iterate (the guess) */
 } pda_implicit_stats;
 typedef struct pda_implicit_s *pda_implicit;
 int pda_implicit_create(pda_problem p, int scheme, pda_implicit *out);
@@ -210,7 +237,9 @@ void pda_implicit_destroy(pda_implicit ip);
 /* A 4-layer 3x3 convolutional network (32 channels, BF16 weights and
    activations, FP32 accumulation on MFMA) predicting the next state of a
    periodic nx x ny two-component field from (y_n, y_{n-1}, nu):
-     guess = 2 y_n - y_{n-1} + s_o * net(y_n, (y_n - y_{n-1})/s_d, (nu - nu_mean)/nu_scale).
+     guess = 2 y_n - y_{n-1} + s_o * net(y_n, (y_n - y_{n-1})/s_d,
+                                          dt f(y_n)/s_f, (nu - nu_mean)/nu_scale)
+   where f(y_n) is the problem's velocity at y_n (d_fn, nullable = 0).
    No counterpart in the reference: it feeds pda_implicit_step's d_guess,
    and Newton on the exact Jacobian still decides the converged state.
    States are device arrays of nx*ny*2 doubles in cell order j*nx + i.
@@ -224,20 +253,21 @@ int pda_guessnet_num_params(pda_guessnet net, int64_t *n);
    resets the optimizer state. */
 int pda_guessnet_set_params(pda_guessnet net, const float *w, int64_t n);
 int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n);
-int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale);
-int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1, double nu,
-                         double *d_guess, pda_stream stream);
+int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale,
+                            double dt_over_sf);
+int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1,
+                         const double *d_fn, double nu, double *d_guess, pda_stream stream);
 /* loss = mean((net - (y_{n+1} - 2 y_n + y_{n-1})/s_o)^2) and its gradient
    w.r.t. the parameters (host arrays, nullable; out = the net's raw
    output, nx*ny*2 floats). */
 int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1,
-                           const double *d_unp1, double nu, float *grad, float *loss,
-                           float *out, pda_stream stream);
+                           const double *d_fn, const double *d_unp1, double nu, float *grad,
+                           float *loss, float *out, pda_stream stream);
 /* one Adam step (beta 0.9 / 0.999) on that loss; loss (nullable) is
    reported before the update */
 int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1,
-                            const double *d_unp1, double nu, float lr, float *loss,
-                            pda_stream stream);
+                            const double *d_fn, const double *d_unp1, double nu, float lr,
+                            float *loss, pda_stream stream);
 
 /* Block view of the Jacobian (m x m blocks).  For each block-row r and block
    slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values

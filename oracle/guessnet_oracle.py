@@ -59,13 +59,16 @@ def _dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def features(un, unm1, nu, H, W, s_d, nu_mean, nu_scale, rb=True):
+def features(un, unm1, nu, H, W, s_d, nu_mean, nu_scale, rb=True, fn=None, dt_over_sf=0.0):
+    """Channels: u, v, du/s_d, dv/s_d, dt f_u/s_f, dt f_v/s_f, nu_hat, 0."""
     u = un.reshape(H, W, 2)
     d = (un - unm1).reshape(H, W, 2) / s_d
     X = np.zeros((H, W, 8))
     X[..., 0:2] = u
     X[..., 2:4] = d
-    X[..., 4] = (nu - nu_mean) / nu_scale
+    if fn is not None:
+        X[..., 4:6] = fn.reshape(H, W, 2) * dt_over_sf
+    X[..., 6] = (nu - nu_mean) / nu_scale
     return bf16(X) if rb else X
 
 
@@ -88,8 +91,9 @@ def forward(w, X0, bf=True):
     return z, cache
 
 
-def loss_and_grad(w, un, unm1, unp1, nu, H, W, s_d, s_o, nu_mean, nu_scale, bf=True):
-    X0 = features(un, unm1, nu, H, W, s_d, nu_mean, nu_scale, rb=bf)
+def loss_and_grad(w, un, unm1, unp1, nu, H, W, s_d, s_o, nu_mean, nu_scale, bf=True, fn=None,
+                  dt_over_sf=0.0):
+    X0 = features(un, unm1, nu, H, W, s_d, nu_mean, nu_scale, rb=bf, fn=fn, dt_over_sf=dt_over_sf)
     out, cache = forward(w, X0, bf)
     t = ((unp1 - 2.0 * un + unm1) / s_o).reshape(H, W, 2)
     if bf:

@@ -27,16 +27,45 @@ using namespace fam;
 // strip) and no workgroup barrier is needed.
 constexpr int VSEG = 63;
 
+// state of cell (i, j) with the index along axis A possibly outside [0, n):
+// periodic wrap, or the mirror cell times the side's ghost sign (the
+// GHOST_MIRROR rule, whose ghost values do not depend on the reading row)
+template <int M, int A>
+__device__ __forceinline__ void load_cell_mapped(const View &P, const Box &B, const double *__restrict__ U,
+                                                 int64_t i, int64_t j, double *q) {
+  int64_t c = A == 0 ? i : j;
+  const int64_t n = B.n[A];
+  int side = -1;
+  if (c < 0) {
+    if (B.mode[A] == 1) c += n;
+    else { c = -1 - c; side = 0; }
+  } else if (c >= n) {
+    if (B.mode[A] == 1) c -= n;
+    else { c = 2 * n - 1 - c; side = 1; }
+  }
+  const int64_t cell = A == 0 ? j * B.n[0] + c : c * B.n[0] + i;
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const double v = U[cell * M + k];
+    q[k] = side < 0 ? v : P.gsign[A][side][k] * v;
+  }
+}
+
+// flux of the face on the low side (along A) of cell (i, j)
 template <class Ph, int W, int A>
-__device__ __forceinline__ void face_flux_fast(const View &P, const double *__restrict__ U,
-                                               int64_t c_right, int64_t stride, double *F) {
+__device__ __forceinline__ void face_flux_fast(const View &P, const Box &B, const double *__restrict__ U,
+                                               int64_t i, int64_t j, double *F) {
   constexpr int M = Ph::M;
+  double st[2 * W][M];
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p)
+    load_cell_mapped<M, A>(P, B, U, A == 0 ? i + p - W : i, A == 1 ? j + p - W : j, st[p]);
   double qm[M], qp[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     double v[2 * W];
 #pragma unroll
-    for (int p = 0; p < 2 * W; ++p) v[p] = U[(c_right + int64_t(p - W) * stride) * M + k];
+    for (int p = 0; p < 2 * W; ++p) v[p] = st[p][k];
     if constexpr (W == 1) {
       qm[k] = v[0];
       qp[k] = v[1];
@@ -50,31 +79,43 @@ __device__ __forceinline__ void face_flux_fast(const View &P, const double *__re
 }
 
 template <class Ph, int W>
-__global__ void __launch_bounds__(64) k_vel_march2d(View P, Box B, int rows_per_strip,
-                                                     const double *__restrict__ U,
-                                                     double *__restrict__ V) {
+__device__ __forceinline__ void vel_march_body(const View &P, const Box &B, int rows_per_strip, int64_t bid,
+                                               const double *__restrict__ U, double *__restrict__ V) {
   constexpr int M = Ph::M;
   const int lane = threadIdx.x;
   const int64_t nx = B.n[0];
   const int64_t bw = B.hi[0] - B.lo[0];
   const int64_t nseg = (bw + VSEG - 1) / VSEG;
-  const int64_t seg = int64_t(blockIdx.x) % nseg, strip = int64_t(blockIdx.x) / nseg;
+  const int64_t seg = bid % nseg, strip = bid / nseg;
   const int64_t iseg = B.lo[0] + seg * VSEG;
   const int ncell = int(bw - seg * VSEG < VSEG ? bw - seg * VSEG : VSEG);
   const int64_t j0 = B.lo[1] + strip * rows_per_strip;
   const int64_t j1 = j0 + rows_per_strip < B.hi[1] ? j0 + rows_per_strip : B.hi[1];
-  // idle lanes repeat the last face; iseg + ncell <= B.hi[0], so every
-  // stencil stays inside the grid
+  // Lane l evaluates the x face on the left of column i = iseg + l; lane
+  // ncell (and the idle lanes after it) the face on the right of the last
+  // cell, i = iseg + ncell <= B.hi[0].  With a full-grid box that column is
+  // n, outside the grid: x stencils there are mapped by the axis mode, and
+  // everything else those lanes compute (y faces, sources; never stored)
+  // uses the last column ic instead.
   const int64_t i = iseg + (lane <= ncell ? lane : ncell);
+  const int64_t ic = i < nx ? i : nx - 1;
   const bool own = lane < ncell;
   const double hx = P.hinv[0], hy = P.hinv[1];
+  // one call site per face kind, so that every face value is the same
+  // whatever the strip height (the first pass only evaluates the face below
+  // the strip)
   double fyb[M];
-  face_flux_fast<Ph, W, 1>(P, U, j0 * nx + i, nx, fyb);
-  for (int64_t j = j0; j < j1; ++j) {
-    const int64_t c = j * nx + i;
-    double fx[M], fy[M], acc[M];
-    face_flux_fast<Ph, W, 0>(P, U, c, 1, fx);
-    face_flux_fast<Ph, W, 1>(P, U, c + nx, nx, fy);
+  for (int64_t j = j0 - 1; j < j1; ++j) {
+    double fy[M];
+    face_flux_fast<Ph, W, 1>(P, B, U, ic, j + 1, fy);
+    if (j < j0) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) fyb[k] = fy[k];
+      continue;
+    }
+    const int64_t c = j * nx + ic;
+    double fx[M], acc[M];
+    face_flux_fast<Ph, W, 0>(P, B, U, i, j, fx);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const double fr = __shfl_down(fx[k], 1, 64);
@@ -88,6 +129,25 @@ __global__ void __launch_bounds__(64) k_vel_march2d(View P, Box B, int rows_per_
       for (int k = 0; k < M; ++k) V[c * M + k] = acc[k];
     }
   }
+}
+
+template <class Ph, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march2d(View P, Box B, int rows_per_strip,
+                                                     const double *__restrict__ U,
+                                                     double *__restrict__ V) {
+  vel_march_body<Ph, W>(P, B, rows_per_strip, int64_t(blockIdx.x), U, V);
+}
+
+// many independent instances (same family and scheme, each with its own
+// grid, parameters and state slice) in one launch: blk_inst maps a
+// workgroup to its instance, whose workgroups are numbered from blk0
+template <class Ph, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march2d_batch(const BatchInst *__restrict__ inst,
+                                                           const int32_t *__restrict__ blk_inst,
+                                                           const double *__restrict__ U,
+                                                           double *__restrict__ V) {
+  const BatchInst &I = inst[blk_inst[blockIdx.x]];
+  vel_march_body<Ph, W>(I.P, I.B, I.rps, int64_t(blockIdx.x) - I.blk0, U + I.off, V + I.off);
 }
 
 // ------------------------------------------------- first-order Jacobian 2D
@@ -408,7 +468,7 @@ bool has_box_kernels(const pda_problem_s &p) {
 
 template <class Ph>
 static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
-  const Box &B = p.box;
+  const Box &B = p.vbox;
   // strips of rows_per_strip rows: ~8k one-wave groups for the 1024^2 grid
   const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
   const int64_t bh = B.hi[1] - B.lo[1];
@@ -423,7 +483,7 @@ static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream
 }
 
 int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
-  if (!p.box.valid || p.box.cells() == 0) return PDA_OK;
+  if (!p.vbox.valid || p.vbox.cells() == 0) return PDA_OK;
   switch (p.family) {
     case PDA_EULER2D: return vel_box<PhEuler2>(p, U, V, s);
     case PDA_SWE2D: return vel_box<PhSwe>(p, U, V, s);
@@ -444,7 +504,7 @@ static int jac_box(const pda_problem_s &p, const double *U, double *J, double *V
       int64_t rps = (bh * nseg) / 16384;
       rps = rps < 1 ? 1 : (rps > 64 ? 64 : rps);
       const int64_t nstrip = (bh + rps - 1) / rps;
-      const View v = generic_view(p, true);
+      const View v = generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size()));
       const int64_t nframe = (v.nvisit + 63) / 64;
       hipLaunchKernelGGL(k_jac_fo_euler2d, dim3(unsigned(nframe + nseg * nstrip)), dim3(64), 0, s, v,
                          B, int(rps), int(nframe), U, J);
@@ -471,6 +531,35 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
     case PDA_BURGERS2D: return jac_box<PhBurgers>(p, U, J, V, s, rest_done);
   }
   return PDA_E_UNSUPPORTED;
+}
+
+// number of one-wave groups the velocity march uses for a box
+int64_t velocity_march_blocks(const Box &B, int rows_per_strip) {
+  const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
+  const int64_t bh = B.hi[1] - B.lo[1];
+  return nseg * ((bh + rows_per_strip - 1) / rows_per_strip);
+}
+
+template <class Ph>
+static void vel_batch(int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks, const double *U,
+                      double *V, hipStream_t s) {
+  const dim3 grid{unsigned(nblocks), 1, 1};
+  if (w == 1) hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 1>), grid, dim3(64), 0, s, inst, blk, U, V);
+  else if (w == 2) hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 2>), grid, dim3(64), 0, s, inst, blk, U, V);
+  else hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 3>), grid, dim3(64), 0, s, inst, blk, U, V);
+}
+
+int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
+                          const double *U, double *V, hipStream_t s) {
+  if (nblocks <= 0) return PDA_OK;
+  switch (family) {
+    case PDA_EULER2D: vel_batch<PhEuler2>(w, inst, blk, nblocks, U, V, s); break;
+    case PDA_SWE2D: vel_batch<PhSwe>(w, inst, blk, nblocks, U, V, s); break;
+    case PDA_BURGERS2D: vel_batch<PhBurgers>(w, inst, blk, nblocks, U, V, s); break;
+    default: set_error("batch velocity: unsupported family"); return PDA_E_UNSUPPORTED;
+  }
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
 }
 
 }  // namespace pda

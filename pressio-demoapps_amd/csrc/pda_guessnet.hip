@@ -5,7 +5,11 @@
 // on the device by the kernels below (forward, backward-data,
 // backward-weight, Adam).
 //
-//   features (8 ch): u_n, v_n, (u_n - u_{n-1})/s_d, (v_n - v_{n-1})/s_d, nu_hat, 0, 0, 0
+//   features (8 ch): u_n, v_n, (u_n - u_{n-1})/s_d, (v_n - v_{n-1})/s_d,
+//                    dt f_u(y_n)/s_f, dt f_v(y_n)/s_f, nu_hat, 0
+//   (f = the problem's velocity at y_n, evaluated by the solver in FP64:
+//   the net receives the time derivative instead of differencing quantised
+//   states)
 //   L1 8->32 SiLU, L2 32->32 SiLU, L3 32->32 SiLU, L4 32->2 (linear)
 //   guess  = 2 y_n - y_{n-1} + s_o * L4          (periodic padding)
 //
@@ -212,19 +216,23 @@ __global__ void k_bias_grad(const __bf16 *__restrict__ dZ, int64_t npix, int cou
 }
 
 // features of the network input
-__global__ void k_features(const double *__restrict__ un, const double *__restrict__ unm1, double nu_hat,
-                           double inv_sd, int64_t npix, __bf16 *__restrict__ X0) {
+__global__ void k_features(const double *__restrict__ un, const double *__restrict__ unm1,
+                           const double *__restrict__ fn, double dt_over_sf, double nu_hat, double inv_sd,
+                           int64_t npix, __bf16 *__restrict__ X0) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= npix) return;
   const double u = un[2 * p], v = un[2 * p + 1];
   const double du = (u - unm1[2 * p]) * inv_sd, dv = (v - unm1[2 * p + 1]) * inv_sd;
+  const double fu = fn ? fn[2 * p] * dt_over_sf : 0., fv = fn ? fn[2 * p + 1] * dt_over_sf : 0.;
   __bf16 *x = X0 + p * CIN0;
   x[0] = __bf16(float(u));
   x[1] = __bf16(float(v));
   x[2] = __bf16(float(du));
   x[3] = __bf16(float(dv));
-  x[4] = __bf16(float(nu_hat));
-  x[5] = x[6] = x[7] = __bf16(0.f);
+  x[4] = __bf16(float(fu));
+  x[5] = __bf16(float(fv));
+  x[6] = __bf16(float(nu_hat));
+  x[7] = __bf16(0.f);
 }
 
 // guess = 2 y_n - y_{n-1} + s_o * out
@@ -315,6 +323,7 @@ struct pda_guessnet_s {
   int H = 0, W = 0;
   int64_t npix = 0;
   double s_d = 1e-2, s_o = 1e-4, nu_mean = 5.5e-3, nu_scale = 4.5e-3;
+  double dt_over_sf = 0.;           // dt / s_f for the velocity features
   int64_t nparam = 0;
   int64_t woff[pda::nn::NL + 1] = {0};
   int64_t foff[pda::nn::NL + 1] = {0}, boff[pda::nn::NL + 1] = {0};
@@ -340,9 +349,11 @@ const float *bias_of(const pda_guessnet_s &net, int l) {
   return net.w.as<float>() + net.woff[l] + 9 * cin_of(l) * cout_of(l);
 }
 
-int features(pda_guessnet_s &net, const double *un, const double *unm1, double nu, hipStream_t st) {
-  hipLaunchKernelGGL(k_features, dim3(unsigned((net.npix + 255) / 256)), dim3(256), 0, st, un, unm1,
-                     (nu - net.nu_mean) / net.nu_scale, 1. / net.s_d, net.npix, net.X0.as<__bf16>());
+int features(pda_guessnet_s &net, const double *un, const double *unm1, const double *fn, double nu,
+             hipStream_t st) {
+  hipLaunchKernelGGL(k_features, dim3(unsigned((net.npix + 255) / 256)), dim3(256), 0, st, un, unm1, fn,
+                     net.dt_over_sf, (nu - net.nu_mean) / net.nu_scale, 1. / net.s_d, net.npix,
+                     net.X0.as<__bf16>());
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
@@ -502,8 +513,10 @@ int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n) {
   return PDA_OK;
 }
 
-int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale) {
-  if (!net || !(s_d > 0.) || !(s_o > 0.) || !(nu_scale > 0.)) return PDA_E_INVALID_ARG;
+int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale,
+                            double dt_over_sf) {
+  if (!net || !(s_d > 0.) || !(s_o > 0.) || !(nu_scale > 0.) || !(dt_over_sf >= 0.)) return PDA_E_INVALID_ARG;
+  net->dt_over_sf = dt_over_sf;
   net->s_d = s_d;
   net->s_o = s_o;
   net->nu_mean = nu_mean;
@@ -511,11 +524,11 @@ int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_
   return PDA_OK;
 }
 
-int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1, double nu,
-                         double *d_guess, pda_stream stream) {
+int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_fn,
+                         double nu, double *d_guess, pda_stream stream) {
   if (!net || !d_un || !d_unm1 || !d_guess) return PDA_E_INVALID_ARG;
   hipStream_t st = hipStream_t(stream);
-  int rc = features(*net, d_un, d_unm1, nu, st);
+  int rc = features(*net, d_un, d_unm1, d_fn, nu, st);
   if (!rc) rc = forward(*net, st);
   if (rc) return rc;
   const int64_t n = net->npix * NOUT;
@@ -525,11 +538,12 @@ int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_u
   return PDA_OK;
 }
 
-int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_unp1,
-                           double nu, float *grad, float *lossv, float *outv, pda_stream stream) {
+int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_fn,
+                           const double *d_unp1, double nu, float *grad, float *lossv, float *outv,
+                           pda_stream stream) {
   if (!net || !d_un || !d_unm1 || !d_unp1) return PDA_E_INVALID_ARG;
   hipStream_t st = hipStream_t(stream);
-  int rc = features(*net, d_un, d_unm1, nu, st);
+  int rc = features(*net, d_un, d_unm1, d_fn, nu, st);
   if (!rc) rc = forward(*net, st);
   if (!rc) rc = loss(*net, d_un, d_unm1, d_unp1, lossv, st);
   if (!rc) rc = backward(*net, st);
@@ -542,11 +556,11 @@ int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d
   return PDA_OK;
 }
 
-int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1,
+int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_fn,
                             const double *d_unp1, double nu, float lr, float *lossv, pda_stream stream) {
   if (!net || !d_un || !d_unm1 || !d_unp1 || !(lr > 0.f)) return PDA_E_INVALID_ARG;
   hipStream_t st = hipStream_t(stream);
-  int rc = features(*net, d_un, d_unm1, nu, st);
+  int rc = features(*net, d_un, d_unm1, d_fn, nu, st);
   if (!rc) rc = forward(*net, st);
   if (!rc) rc = loss(*net, d_un, d_unm1, d_unp1, lossv, st);
   if (!rc) rc = backward(*net, st);

@@ -565,6 +565,7 @@ int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev, const
       hipLaunchKernelGGL(k_comb, dim3(gs), dim3(TB), 0, st, R, Y, cnp1, Yn, cn, Rx2, c2, F, cf, n);
       PDA_HIP_CHECK(hipGetLastError());
       S.residual_norm = norm2(*ip, R, st);
+      if (it == 0) S.initial_residual_norm = S.residual_norm;
       if (S.residual_norm <= ip->tol) {
         S.converged = 1;
         break;
@@ -579,6 +580,7 @@ int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev, const
       hipLaunchKernelGGL(k_comb, dim3(gs), dim3(TB), 0, st, R, Y, cnp1, Yn, cn, Rx2, c2, F, cf, n);
       PDA_HIP_CHECK(hipGetLastError());
       S.residual_norm = norm2(*ip, R, st);
+      if (it == 0) S.initial_residual_norm = S.residual_norm;
     }
     // A = cnp1 I + cf df/dy; solve A dx = -R
     hipLaunchKernelGGL(k_scale, dim3(grid_strided(p.nnz)), dim3(TB), 0, st, J, p.nnz, cf);

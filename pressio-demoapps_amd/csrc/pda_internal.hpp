@@ -132,6 +132,10 @@ struct View {
 struct Box {
   int64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0}; // half-open, per axis
   int64_t n[3] = {1, 1, 1};                     // grid extent
+  // how a stencil index outside [0, n) along the axis is read:
+  // 0 never happens (the box keeps stencils inside), 1 periodic wrap,
+  // 2 mirror image about the boundary times the side's ghost sign
+  int mode[3] = {0, 0, 0};
   bool valid = false;
   int64_t cells() const {
     return valid ? (hi[0] - lo[0]) * (hi[1] - lo[1]) * (hi[2] - lo[2]) : 0;
@@ -139,6 +143,16 @@ struct Box {
 };
 
 enum GhostRule { GHOST_NONE = 0, GHOST_OPPOSITE = 1, GHOST_MIRROR = 2 };
+
+// one instance of a batched evaluation: its problem view and velocity box,
+// the offset of its state slice, its first workgroup and strip height
+struct BatchInst {
+  View P;
+  Box B;
+  int64_t off = 0;
+  int64_t blk0 = 0;
+  int rps = 1;
+};
 
 }  // namespace pda
 
@@ -163,6 +177,12 @@ struct pda_problem_s {
   pda::Box box;
   std::vector<int32_t> h_rem_rows;   // rows outside the box
   pda::DevBuf d_rem_rows;
+  // velocity box: with periodic or mirror-ghost axes it spans the whole
+  // grid (ghost values of those rules do not depend on the row reading
+  // them), else the same interior box
+  pda::Box vbox;
+  std::vector<int32_t> h_vrem_rows;  // rows outside the velocity box
+  pda::DevBuf d_vrem_rows;
   int use_box = 1;                   // 0 forces the generic kernels everywhere
   pda::View view;
   pda::DevBuf d_graph, d_ghost_row, d_near_rows, d_ghost, d_bcf;
@@ -185,7 +205,10 @@ int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipS
 // rest_done: the launch also covered the rows outside the box
 int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, double *V,
                         hipStream_t s, bool *rest_done);
-// view of the generic kernels: every row, or only the rows outside the box
-View generic_view(const pda_problem_s &p, bool box_done);
+int64_t velocity_march_blocks(const Box &B, int rows_per_strip);
+int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
+                          const double *U, double *V, hipStream_t s);
+// view of the generic kernels: every row (rows == nullptr) or a row list
+View generic_view(const pda_problem_s &p, const pda::DevBuf *rows, int64_t count);
 bool has_box_kernels(const pda_problem_s &p);
 }  // namespace pda

@@ -81,11 +81,11 @@ __global__ void k_fill(double *x, int64_t n, double v) {
 }
 
 // the rows the generic kernels cover: all of them, or those outside the box
-View generic_view(const pda_problem_s &p, bool box_done) {
+View generic_view(const pda_problem_s &p, const DevBuf *rows, int64_t count) {
   View v = p.view;
-  if (box_done) {
-    v.rows = p.d_rem_rows.as<int32_t>();
-    v.nvisit = int64_t(p.h_rem_rows.size());
+  if (rows) {
+    v.rows = rows->as<int32_t>();
+    v.nvisit = count;
   } else {
     v.rows = nullptr;
     v.nvisit = v.nrows;
@@ -118,10 +118,11 @@ static int jac_generic(const View &v, const pda_problem_s &p, const double *U, d
 }
 
 static bool box_velocity(const pda_problem_s &p) {
-  return p.use_box && p.box.valid && p.box.cells() > 0 && has_box_kernels(p);
+  return p.use_box && p.vbox.valid && p.vbox.cells() > 0 && has_box_kernels(p);
 }
 static bool box_jacobian(const pda_problem_s &p) {
-  return box_velocity(p) && !p.view.dup && p.w == 1;
+  return p.use_box && p.box.valid && p.box.cells() > 0 && has_box_kernels(p) && !p.view.dup &&
+         p.w == 1;
 }
 
 // With a structured box the box kernel and the generic kernel over the rows
@@ -129,8 +130,10 @@ static bool box_jacobian(const pda_problem_s &p) {
 // a box launcher may take the outside rows into its own launch.
 int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, hipStream_t s) {
   if (p.view.nrows == 0) return PDA_OK;
-  if (!box_velocity(p)) return vel_generic(generic_view(p, false), p, U, V, s);
-  int rc = p.h_rem_rows.empty() ? PDA_OK : vel_generic(generic_view(p, true), p, U, V, s);
+  if (!box_velocity(p)) return vel_generic(generic_view(p, nullptr, 0), p, U, V, s);
+  int rc = p.h_vrem_rows.empty()
+               ? PDA_OK
+               : vel_generic(generic_view(p, &p.d_vrem_rows, int64_t(p.h_vrem_rows.size())), p, U, V, s);
   if (rc) return rc;
   return launch_velocity_box(p, U, V, s);
 }
@@ -143,12 +146,12 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
       hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, J, p.nnz, 0.);
       PDA_HIP_CHECK(hipGetLastError());
     }
-    return jac_generic(generic_view(p, false), p, U, V, J, s);
+    return jac_generic(generic_view(p, nullptr, 0), p, U, V, J, s);
   }
   bool rest_done = false;
   int rc = launch_jacobian_box(p, U, J, V, s, &rest_done);
   if (rc || rest_done || p.h_rem_rows.empty()) return rc;
-  return jac_generic(generic_view(p, true), p, U, V, J, s);
+  return jac_generic(generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size())), p, U, V, J, s);
 }
 
 int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64_t ncols,

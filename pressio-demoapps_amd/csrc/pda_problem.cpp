@@ -359,6 +359,25 @@ int build(pda_problem_s &p) {
         if (i < p.box.lo[0] || i >= p.box.hi[0] || j < p.box.lo[1] || j >= p.box.hi[1])
           p.h_rem_rows.push_back(int32_t(r));
       }
+      // velocity box
+      p.vbox = p.box;
+      for (int a = 0; a < dim; ++a) {
+        const bool per = (M.periodic_mask >> a) & 1;
+        const bool big = M.n[a] >= 2 * w + 1;
+        if (big && per) p.vbox.mode[a] = 1;
+        else if (big && !per && p.view.ghost_rule == GHOST_MIRROR && p.family != PDA_BURGERS2D)
+          p.vbox.mode[a] = 2;
+        if (p.vbox.mode[a]) {
+          p.vbox.lo[a] = 0;
+          p.vbox.hi[a] = M.n[a];
+        }
+      }
+      p.h_vrem_rows.clear();
+      for (int64_t r = 0; r < nr; ++r) {
+        const int64_t i = r % M.n[0], j = r / M.n[0];
+        if (i < p.vbox.lo[0] || i >= p.vbox.hi[0] || j < p.vbox.lo[1] || j >= p.vbox.hi[1])
+          p.h_vrem_rows.push_back(int32_t(r));
+      }
     }
   }
   p.h_ghost_row = std::move(ghost_row);
@@ -440,6 +459,7 @@ int ensure_device(pda_problem_s &p) {
   PDA_HIP_CHECK(p.d_slot.upload(p.h_slot));
   PDA_HIP_CHECK(p.d_bcol.upload(p.h_bcol));
   PDA_HIP_CHECK(p.d_rem_rows.upload(p.h_rem_rows));
+  PDA_HIP_CHECK(p.d_vrem_rows.upload(p.h_vrem_rows));
   const size_t gn = size_t(2 * p.dim) * size_t(std::max<int64_t>(p.n_near, 1)) * p.w * p.m;
   // unused ghost slots hold numeric_limits<double>::min(),
   // impl/euler_2d_prob_class.hpp:1266-1270
@@ -526,12 +546,12 @@ int pda_problem_create(pda_mesh mesh, int family, int prob, int scheme, int ic_f
   if (family == PDA_EULER1D || family == PDA_EULER3D) p->phys[0] = 1.4;
   p->mesh = mesh;
   mesh_retain(mesh);
+  p->view.ghost_rule = rule;
   const int rc = build(*p);
   if (rc) {
     pda_problem_destroy(p);
     return rc;
   }
-  p->view.ghost_rule = rule;
   *out = p;
   return PDA_OK;
 }

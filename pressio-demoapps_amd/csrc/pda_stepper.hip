@@ -11,7 +11,7 @@
 // stream; the capture is redone when the state pointer or dt changes.  The
 // evaluation kernels use no memsets, so the captured step has only kernel
 // nodes.
-#include "pda_internal.hpp"
+#include "pda_rk.hpp"
 
 namespace pda {
 namespace {
@@ -33,6 +33,8 @@ __global__ void __launch_bounds__(256) k_lincomb(double *out, Lin L, int64_t len
   }
 }
 
+}  // namespace
+
 int lincomb(double *out, std::initializer_list<std::pair<double, const double *>> terms, int64_t len,
             hipStream_t s) {
   Lin L{};
@@ -49,7 +51,6 @@ int lincomb(double *out, std::initializer_list<std::pair<double, const double *>
   return PDA_OK;
 }
 
-}  // namespace
 }  // namespace pda
 
 struct pda_stepper_s {
@@ -75,59 +76,10 @@ using namespace pda;
 // one step U <- step(U) at time t; buffers of the stepper
 int step_once(pda_stepper_s &st, double *U, double t, double dt, hipStream_t s) {
   const pda_problem_s &p = *st.prob;
-  const int64_t n = st.len;
-  double *aux = st.aux.as<double>();
-  double *k1 = st.k[0].as<double>(), *k2 = st.k[1].as<double>();
-  double *k3 = st.k[2].as<double>(), *k4 = st.k[3].as<double>();
-  int rc;
-#define PDA_TRY(x) \
-  do {             \
-    rc = (x);      \
-    if (rc) return rc; \
-  } while (0)
-  switch (st.scheme) {
-    case PDA_FORWARD_EULER:
-      // y_{n+1} = y_n + dt f(y_n)
-      PDA_TRY(launch_velocity(p, U, t, k1, s));
-      PDA_TRY(lincomb(U, {{1., U}, {dt, k1}}, n, s));
-      break;
-    case PDA_SSPRK3: {
-      const double third = 1. / 3., twoThird = 2. / 3., quarter = 1. / 4., threeQuarter = 3. / 4.;
-      PDA_TRY(launch_velocity(p, U, t, k1, s));
-      PDA_TRY(lincomb(aux, {{1., U}, {dt, k1}}, n, s));                       // u1 = u + dt f(u)
-      PDA_TRY(launch_velocity(p, aux, t + dt, k1, s));
-      PDA_TRY(lincomb(aux, {{quarter, aux}, {threeQuarter, U}, {quarter * dt, k1}}, n, s));
-      PDA_TRY(launch_velocity(p, aux, t + dt / 2., k1, s));
-      PDA_TRY(lincomb(U, {{third, U}, {twoThird, aux}, {twoThird * dt, k1}}, n, s));
-      break;
-    }
-    case PDA_RK4: {
-      const double h2 = dt / 2., h6 = dt / 6., h3 = dt / 3.;
-      PDA_TRY(launch_velocity(p, U, t, k1, s));
-      PDA_TRY(lincomb(aux, {{1., U}, {h2, k1}}, n, s));
-      PDA_TRY(launch_velocity(p, aux, t + h2, k2, s));
-      PDA_TRY(lincomb(aux, {{1., U}, {h2, k2}}, n, s));
-      PDA_TRY(launch_velocity(p, aux, t + h2, k3, s));
-      PDA_TRY(lincomb(aux, {{1., U}, {dt, k3}}, n, s));
-      PDA_TRY(launch_velocity(p, aux, t + dt, k4, s));
-      PDA_TRY(lincomb(U, {{1., U}, {h6, k1}, {h3, k2}, {h3, k3}, {h6, k4}}, n, s));
-      break;
-    }
-    case PDA_RK2: {
-      // pressiodemoapps/__init__.py:90-114: k1 = dt f(y), k2 = dt f(y + k1),
-      // y += k2/2 + k1/2
-      PDA_TRY(launch_velocity(p, U, t, k1, s));
-      PDA_TRY(lincomb(aux, {{1., U}, {dt, k1}}, n, s));
-      PDA_TRY(launch_velocity(p, aux, t + dt, k2, s));
-      PDA_TRY(lincomb(U, {{1., U}, {0.5 * dt, k2}, {0.5 * dt, k1}}, n, s));
-      break;
-    }
-    default:
-      set_error("stepper: unknown scheme");
-      return PDA_E_INVALID_ARG;
-  }
-#undef PDA_TRY
-  return PDA_OK;
+  double *k[4] = {st.k[0].as<double>(), st.k[1].as<double>(), st.k[2].as<double>(), st.k[3].as<double>()};
+  return rk_step(
+      st.scheme, [&](const double *y, double tt, double *f) { return launch_velocity(p, y, tt, f, s); }, U,
+      st.aux.as<double>(), k, st.len, t, dt, s);
 }
 
 }  // namespace

@@ -33,7 +33,8 @@ class ImplicitStats(ctypes.Structure):
     _fields_ = [("newton_iters", ctypes.c_int), ("linear_iters", ctypes.c_int),
                 ("converged", ctypes.c_int), ("residual_norm", ctypes.c_double),
                 ("correction_norm", ctypes.c_double),
-                ("last_linear_rel_residual", ctypes.c_double)]
+                ("last_linear_rel_residual", ctypes.c_double),
+                ("initial_residual_norm", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -45,6 +46,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(_LIBPATH):
             raise PdaError(f"libpda.so not built at {_LIBPATH}; run __graft_entry__.build()")
+        # torch owns the process's HIP runtime (device memory, streams); let it
+        # initialise before the library's first HIP call so both use one
+        # runtime instance
+        try:
+            import torch
+            torch.cuda.is_available()
+        except ImportError:
+            pass
         L = ctypes.CDLL(_LIBPATH)
         i32, i64, dbl, vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
         P = ctypes.POINTER
@@ -85,11 +94,19 @@ def lib():
             "pda_guessnet_num_params": ([vp, P(i64)], i32),
             "pda_guessnet_set_params": ([vp, P(ctypes.c_float), i64], i32),
             "pda_guessnet_get_params": ([vp, P(ctypes.c_float), i64], i32),
-            "pda_guessnet_set_scales": ([vp, dbl, dbl, dbl, dbl], i32),
-            "pda_guessnet_predict": ([vp, vp, vp, dbl, vp, vp], i32),
-            "pda_guessnet_gradients": ([vp, vp, vp, vp, dbl, P(ctypes.c_float), P(ctypes.c_float),
+            "pda_guessnet_set_scales": ([vp, dbl, dbl, dbl, dbl, dbl], i32),
+            "pda_guessnet_predict": ([vp, vp, vp, vp, dbl, vp, vp], i32),
+            "pda_guessnet_gradients": ([vp, vp, vp, vp, vp, dbl, P(ctypes.c_float), P(ctypes.c_float),
                                         P(ctypes.c_float), vp], i32),
-            "pda_guessnet_train_step": ([vp, vp, vp, vp, dbl, ctypes.c_float, P(ctypes.c_float), vp], i32),
+            "pda_guessnet_train_step": ([vp, vp, vp, vp, vp, dbl, ctypes.c_float, P(ctypes.c_float), vp],
+                                        i32),
+            "pda_batch_create": ([P(vp), i32, P(vp)], i32),
+            "pda_batch_destroy": ([vp], None),
+            "pda_batch_sizes": ([vp, P(i64), P(i64)], i32),
+            "pda_batch_initial_condition": ([vp, vp, vp], i32),
+            "pda_batch_velocity": ([vp, vp, dbl, vp, vp], i32),
+            "pda_batch_autotune": ([vp, vp, vp, P(i32), P(dbl)], i32),
+            "pda_batch_advance": ([vp, i32, vp, dbl, dbl, i64, i32, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -577,8 +594,9 @@ class ImplicitStepper:
 class GuessNet:
     """Learned initial guess for implicit steps of a periodic nx x ny
     two-component field (pda_guessnet_*): guess = 2 y_n - y_{n-1} +
-    s_o * net(y_n, (y_n - y_{n-1})/s_d, nu_hat), a 4-layer 3x3 conv net with
-    BF16 weights and FP32 accumulation on MFMA, trained on the device."""
+    s_o * net(y_n, (y_n - y_{n-1})/s_d, dt f(y_n)/s_f, nu_hat), a 4-layer
+    3x3 conv net with BF16 weights and FP32 accumulation on MFMA, trained on
+    the device.  fn = f(y_n), the problem's velocity at y_n (optional)."""
 
     def __init__(self, nx, ny):
         h = ctypes.c_void_p()
@@ -588,7 +606,7 @@ class GuessNet:
         n = ctypes.c_int64()
         _check(lib().pda_guessnet_num_params(self._h, ctypes.byref(n)))
         self.nparams = n.value
-        self.scales = (1e-2, 1e-4, 5.5e-3, 4.5e-3)
+        self.scales = (1e-2, 1e-4, 5.5e-3, 4.5e-3, 0.0)
 
     def set_params(self, w):
         w = np.ascontiguousarray(w, dtype=np.float32)
@@ -601,35 +619,96 @@ class GuessNet:
                                              len(w)))
         return w
 
-    def set_scales(self, s_d, s_o, nu_mean, nu_scale):
-        self.scales = (float(s_d), float(s_o), float(nu_mean), float(nu_scale))
+    def set_scales(self, s_d, s_o, nu_mean, nu_scale, dt_over_sf=0.0):
+        self.scales = (float(s_d), float(s_o), float(nu_mean), float(nu_scale), float(dt_over_sf))
         _check(lib().pda_guessnet_set_scales(self._h, *self.scales))
 
-    def predict(self, un, unm1, nu, out=None):
+    @staticmethod
+    def _opt(t):
+        return _ptr(t) if t is not None else None
+
+    def predict(self, un, unm1, nu, out=None, fn=None):
         if out is None:
             out = _torch().empty_like(un)
-        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), float(nu), _ptr(out), _stream()))
+        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), self._opt(fn), float(nu), _ptr(out),
+                                          _stream()))
         return out
 
-    def gradients(self, un, unm1, unp1, nu):
+    def gradients(self, un, unm1, unp1, nu, fn=None):
         g = np.zeros(self.nparams, dtype=np.float32)
         o = np.zeros(self.nx * self.ny * 2, dtype=np.float32)
         loss = ctypes.c_float()
         F = ctypes.POINTER(ctypes.c_float)
-        _check(lib().pda_guessnet_gradients(self._h, _ptr(un), _ptr(unm1), _ptr(unp1), float(nu),
-                                            g.ctypes.data_as(F), ctypes.byref(loss), o.ctypes.data_as(F),
-                                            _stream()))
+        _check(lib().pda_guessnet_gradients(self._h, _ptr(un), _ptr(unm1), self._opt(fn), _ptr(unp1),
+                                            float(nu), g.ctypes.data_as(F), ctypes.byref(loss),
+                                            o.ctypes.data_as(F), _stream()))
         return loss.value, g, o
 
-    def train_step(self, un, unm1, unp1, nu, lr, want_loss=False):
+    def train_step(self, un, unm1, unp1, nu, lr, want_loss=False, fn=None):
         loss = ctypes.c_float()
-        _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), _ptr(unp1), float(nu),
-                                             float(lr), ctypes.byref(loss) if want_loss else None,
-                                             _stream()))
+        _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), self._opt(fn), _ptr(unp1),
+                                             float(nu), float(lr),
+                                             ctypes.byref(loss) if want_loss else None, _stream()))
         return loss.value if want_loss else None
 
     def __del__(self):
         h = getattr(self, "_h", None)
         if h and _lib is not None:
             _lib.pda_guessnet_destroy(h)
+            self._h = None
+
+
+def _guessnet_layers():
+    """(cin, cout) of the guess network's layers (pda_guessnet_set_params)."""
+    return [(8, 32), (32, 32), (32, 32), (32, 2)]
+
+
+# ----------------------------------------------------------------- batching
+class Batch:
+    """Many independent 2D instances of one family and scheme evaluated and
+    stepped together (pda_batch_*): the state is the concatenation of the
+    instances' states; one launch evaluates every instance's velocity."""
+
+    def __init__(self, problems):
+        self._probs = list(problems)  # the batch references them
+        arr = (ctypes.c_void_p * len(self._probs))(*[p.handle for p in self._probs])
+        h = ctypes.c_void_p()
+        _check(lib().pda_batch_create(arr, len(self._probs), ctypes.byref(h)))
+        self._h = h.value
+        tot = ctypes.c_int64()
+        offs = (ctypes.c_int64 * (len(self._probs) + 1))()
+        _check(lib().pda_batch_sizes(self._h, ctypes.byref(tot), offs))
+        self.total = tot.value
+        self.offsets = np.array(offs[:], dtype=np.int64)
+
+    def initialCondition(self):
+        U = _torch().empty(self.total, dtype=_torch().float64, device="cuda")
+        _check(lib().pda_batch_initial_condition(self._h, _ptr(U), _stream()))
+        return U
+
+    def createRightHandSide(self):
+        return _torch().zeros(self.total, dtype=_torch().float64, device="cuda")
+
+    def rightHandSide(self, U, t, V):
+        _dev_tensor(U, self.total, "state")
+        _dev_tensor(V, self.total, "velocity")
+        _check(lib().pda_batch_velocity(self._h, _ptr(U), float(t), _ptr(V), _stream()))
+
+    def autotune(self, U):
+        r, us = ctypes.c_int(), ctypes.c_double()
+        _check(lib().pda_batch_autotune(self._h, _ptr(U), _stream(), ctypes.byref(r), ctypes.byref(us)))
+        return r.value, us.value
+
+    def advance(self, U, scheme, dt, nsteps, t0=0.0, steps_per_graph=10):
+        _dev_tensor(U, self.total, "state")
+        _check(lib().pda_batch_advance(self._h, int(scheme), _ptr(U), float(t0), float(dt), int(nsteps),
+                                       int(steps_per_graph), _stream()))
+
+    def instance(self, U, i):
+        return U[self.offsets[i]:self.offsets[i + 1]]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pda_batch_destroy(h)
             self._h = None

@@ -1,0 +1,230 @@
+"""Learned initial guesses for implicit Newton steps of periodic 2D viscous
+Burgers (WENO5, BDF2), trained on trajectories this package's own solver
+generates, with the device network of GuessNet.
+
+Workload: 512 x 512 cells on [-1, 1]^2, diffusion nu ~ U(1e-3, 1e-2), Gaussian
+pulse of amplitude U(0.5, 1.5) centred at U(-0.5, 0.5)^2 (spread 0.15), BDF2
+with dt = 5e-3 (first step BDF1); Newton on the exact WENO5 Jacobian to
+||R||_2 <= 1e-10 decides every step, whatever the guess.
+
+    python -m pressiodemoapps_amd.learned train --out weights.npz   # generate data, train
+    python -m pressiodemoapps_amd.learned compare --weights weights.npz
+
+The weights file is a plain .npz (float32 parameters and float64 scales),
+loaded without pickle.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import (AdvectionDiffusion2d, GuessNet, ImplicitScheme, ImplicitStepper,
+               InviscidFluxReconstruction, StopRule, create_problem, create_structured_mesh)
+
+DT = 5e-3
+TOL = 1e-10
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                               "burgers_guessnet_512.npz")
+
+
+def draw_params(rng):
+    """One instance of the workload's parameter distribution."""
+    return {"diffusion": float(rng.uniform(1e-3, 1e-2)),
+            "pulseMagnitude": float(rng.uniform(0.5, 1.5)),
+            "pulseX": float(rng.uniform(-0.5, 0.5)),
+            "pulseY": float(rng.uniform(-0.5, 0.5)),
+            "pulseSpread": 0.15}
+
+
+def make_problem(mesh, params):
+    return create_problem(mesh, AdvectionDiffusion2d.BurgersPeriodic, InviscidFluxReconstruction.Weno5,
+                          1, dict(params))
+
+
+def burgers_mesh(n):
+    return create_structured_mesh((n, n), (-1.0, 1.0, -1.0, 1.0), 7, True)
+
+
+def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_step=None):
+    """BDF2 trajectory from the IC.  guess: None (previous step, y_n),
+    "extrap" (2 y_n - y_{n-1}) or a GuessNet.  keep: set of step indices k
+    whose (y_{k-1}, y_k, y_{k+1}) triples are returned.  Returns
+    (final state, per-step stats, kept triples)."""
+    import torch
+    ip = ImplicitStepper(prob, ImplicitScheme.BDF2)
+    ip.set_options(StopRule.Residual, tol, 30, 1e-12, 0)
+    U = prob.initialCondition()
+    prev = None
+    stats, triples = [], []
+    g = torch.empty_like(U)
+    fn = torch.empty_like(U)
+    for k in range(nsteps):
+        yn = U.clone()
+        gs = None
+        if prev is not None and guess is not None:
+            if isinstance(guess, str):
+                torch.mul(yn, 2.0, out=g)
+                g.sub_(prev)
+            else:
+                prob.rightHandSide(yn, k * DT, fn)
+                guess.predict(yn, prev, nu, out=g, fn=fn)
+            gs = g
+        st = ip.step(U, k * DT, DT, prev=prev, guess=gs)
+        stats.append(st)
+        if keep is not None and k in keep and prev is not None:
+            f = torch.empty_like(U)
+            prob.rightHandSide(yn, k * DT, f)
+            triples.append((prev.clone(), yn.clone(), U.clone(), f))
+        if on_step:
+            on_step(k, st)
+        prev = yn
+    return U, stats, triples
+
+
+def generate(n, n_traj, nsteps, keep_every, seed, log=print):
+    """Training data: every keep_every-th (y_{k-1}, y_k, y_{k+1}) of n_traj
+    trajectories; list of (nu, triple)."""
+    rng = np.random.default_rng(seed)
+    mesh = burgers_mesh(n)
+    data = []
+    keep = set(range(1, nsteps, keep_every))
+    t0 = time.perf_counter()
+    for i in range(n_traj):
+        params = draw_params(rng)
+        prob = make_problem(mesh, params)
+        _, stats, tr = run_trajectory(prob, nsteps, keep=keep)
+        data += [(params["diffusion"], t) for t in tr]
+        log(f"[learned] trajectory {i + 1}/{n_traj} nu={params['diffusion']:.4f}: {len(tr)} samples, "
+            f"{np.mean([s['newton_iters'] for s in stats]):.2f} Newton/step, "
+            f"{time.perf_counter() - t0:.0f} s")
+    return data
+
+
+def init_params(net, seed):
+    """He-style initial weights, small last layer."""
+    from . import _guessnet_layers
+    rng = np.random.default_rng(seed)
+    w = np.zeros(net.nparams, dtype=np.float32)
+    o = 0
+    layers = _guessnet_layers()
+    for li, (ci, co) in enumerate(layers):
+        nW = 9 * ci * co
+        std = np.sqrt(2.0 / (9 * ci)) * (0.1 if li == len(layers) - 1 else 1.0)
+        w[o:o + nW] = rng.standard_normal(nW) * std
+        o += nW + co
+    net.set_params(w)
+
+
+def fit(net, data, steps, lr, seed, log=print):
+    """Adam on random samples, cosine learning-rate decay."""
+    rng = np.random.default_rng(seed)
+    losses = []
+    for s in range(steps):
+        nu, (ym1, y0, yp1, f0) = data[int(rng.integers(len(data)))]
+        lr_s = lr * 0.5 * (1.0 + np.cos(np.pi * s / steps))
+        want = (s % 100 == 0) or s == steps - 1
+        loss = net.train_step(y0, ym1, yp1, nu, max(lr_s, 1e-6), want_loss=want, fn=f0)
+        if want:
+            losses.append((s, loss))
+            log(f"[learned] step {s}: loss {loss:.4e}")
+    return losses
+
+
+def scales_from(data):
+    """(s_d, s_o, s_f): spreads of y_n - y_{n-1}, of the target
+    y_{n+1} - 2 y_n + y_{n-1} and of dt f(y_n) over the data."""
+    d = np.concatenate([(t[1] - t[0]).flatten()[::97].cpu().numpy() for _, t in data])
+    o = np.concatenate([(t[2] - 2 * t[1] + t[0]).flatten()[::97].cpu().numpy() for _, t in data])
+    f = np.concatenate([(DT * t[3]).flatten()[::97].cpu().numpy() for _, t in data])
+    return float(np.std(d)), float(np.std(o)), float(np.std(f))
+
+
+def save(path, net, meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=net.get_params(), scales=np.asarray(net.scales, dtype=np.float64),
+             meta=np.asarray(json.dumps(meta)))
+
+
+def load(path, n):
+    z = np.load(path, allow_pickle=False)
+    net = GuessNet(n, n)
+    net.set_scales(*[float(v) for v in z["scales"]])
+    net.set_params(z["params"])
+    return net, json.loads(str(z["meta"]))
+
+
+def compare(net, n, nsteps, seed, log=print):
+    """Same held-out instance stepped with the previous-step guess, the
+    linear extrapolation and the network; every run converges each step to
+    ||R|| <= TOL.  Returns a dict of steps/s, Newton and linear iterations,
+    and the final-state differences to the previous-step (cold start) run."""
+    import torch
+    rng = np.random.default_rng(seed)
+    params = draw_params(rng)
+    mesh = burgers_mesh(n)
+    prob = make_problem(mesh, params)
+    out = {"params": params, "steps": nsteps}
+    finals = {}
+    for name, g in (("previous_step", None), ("extrapolation", "extrap"), ("learned", net)):
+        # warm-up (library set-up, first-touch allocations)
+        run_trajectory(prob, 2, guess=g, nu=params["diffusion"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        U, stats, _ = run_trajectory(prob, nsteps, guess=g, nu=params["diffusion"])
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        finals[name] = U.cpu().numpy()
+        out[name] = {"steps_per_s": nsteps / el,
+                     "newton_per_step": float(np.mean([s["newton_iters"] for s in stats])),
+                     "linear_per_step": float(np.mean([s["linear_iters"] for s in stats])),
+                     "median_initial_residual": float(np.median([s["initial_residual_norm"] for s in stats[2:]])),
+                     "max_final_residual": float(max(s["residual_norm"] for s in stats))}
+        log(f"[learned] {name}: {out[name]}")
+    ref = finals["previous_step"]
+    for name in ("extrapolation", "learned"):
+        out[name]["final_state_max_abs_diff_vs_cold_start"] = float(np.abs(finals[name] - ref).max())
+    out["speedup_learned_vs_previous_step"] = out["learned"]["steps_per_s"] / out["previous_step"]["steps_per_s"]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="pressiodemoapps_amd.learned")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--n", type=int, default=512)
+    t.add_argument("--trajectories", type=int, default=64)
+    t.add_argument("--steps", type=int, default=400)
+    t.add_argument("--keep-every", type=int, default=25)
+    t.add_argument("--train-steps", type=int, default=3000)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=2024)
+    t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    c = sub.add_parser("compare")
+    c.add_argument("--n", type=int, default=512)
+    c.add_argument("--steps", type=int, default=400)
+    c.add_argument("--seed", type=int, default=77)
+    c.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.cmd == "train":
+        data = generate(a.n, a.trajectories, a.steps, a.keep_every, a.seed, log)
+        s_d, s_o, s_f = scales_from(data)
+        net = GuessNet(a.n, a.n)
+        net.set_scales(s_d, s_o, 5.5e-3, 4.5e-3, DT / s_f)
+        init_params(net, a.seed + 1)
+        losses = fit(net, data, a.train_steps, a.lr, a.seed + 2, log)
+        meta = {"n": a.n, "trajectories": a.trajectories, "steps": a.steps, "keep_every": a.keep_every,
+                "samples": len(data), "train_steps": a.train_steps, "lr": a.lr, "seed": a.seed,
+                "s_d": s_d, "s_o": s_o, "s_f": s_f, "final_loss": losses[-1][1]}
+        save(a.out, net, meta)
+        print(json.dumps(meta))
+    else:
+        net, meta = load(a.weights, a.n)
+        print(json.dumps(compare(net, a.n, a.steps, a.seed, log)))
+
+
+if __name__ == "__main__":
+    main()

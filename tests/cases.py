@@ -25,6 +25,13 @@ for s in (S.FirstOrder, S.Weno3):
         (f"euler3d-periodic-{s.name}", pda.Euler3d.PeriodicSmooth, s, (8, 8, 8), (-1, 1, -1, 1, -1, 1), 5, True, 0, None),
     ]
 
+# 3D WENO5 (build-defined: the reference stops at WENO3 in 3D); pinned by
+# conservation and axis-permutation equivariance in test_oracle_kat.py
+CASES += [
+    ("euler3d-sedov-Weno5", pda.Euler3d.SedovSymmetry, S.Weno5, (10, 9, 8), (0, 1.2, 0, 1.2, 0, 1.2), 7, (), 0, None),
+    ("euler3d-periodic-Weno5", pda.Euler3d.PeriodicSmooth, S.Weno5, (8, 8, 8), (-1, 1, -1, 1, -1, 1), 7, True, 0, None),
+]
+
 IDS = [c[0] for c in CASES]
 
 # larger meshes: several 64-wide tiles, partial segments, tall and wide boxes

@@ -41,11 +41,13 @@ def test_forward_and_gradients_match_oracle(torch):
     w = (rng.standard_normal(net.nparams) * 0.15).astype(np.float32)
     net.set_params(w)
     scales = (1e-2, 1e-3, 5.5e-3, 4.5e-3)
-    net.set_scales(*scales)
+    net.set_scales(*scales, 0.7)
     un, unm1, unp1 = _fields(rng, H, W)
+    fn = rng.standard_normal(un.shape)
     t = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
-    loss, g, out = net.gradients(t(un), t(unm1), t(unp1), 0.004)
-    lo, oo, go = G.loss_and_grad(w.astype(np.float64), un, unm1, unp1, 0.004, H, W, *scales, bf=True)
+    loss, g, out = net.gradients(t(un), t(unm1), t(unp1), 0.004, fn=t(fn))
+    lo, oo, go = G.loss_and_grad(w.astype(np.float64), un, unm1, unp1, 0.004, H, W, *scales, bf=True, fn=fn,
+                                 dt_over_sf=0.7)
     print(f"[gpu-nn] out {rel(out, oo):.2e} loss {abs(loss - lo) / lo:.2e}", flush=True)
     assert rel(out, oo) <= 1e-2
     assert abs(loss - lo) <= 1e-2 * lo

@@ -24,14 +24,16 @@ def test_oracle_gradient_matches_finite_differences():
     unm1 = un + 0.01 * rng.standard_normal(H * W * 2)
     unp1 = un + 0.02 * rng.standard_normal(H * W * 2)
     args = (un, unm1, unp1, 0.004, H, W, 1e-2, 1e-2, 5.5e-3, 4.5e-3)
-    _, _, g = G.loss_and_grad(w, *args, bf=False)
+    fn = rng.standard_normal(H * W * 2)
+    kw = {"fn": fn, "dt_over_sf": 0.5}
+    _, _, g = G.loss_and_grad(w, *args, bf=False, **kw)
     idx = rng.choice(len(w), size=60, replace=False)
     h = 1e-6
     for i in idx:
         wp, wm = w.copy(), w.copy()
         wp[i] += h
         wm[i] -= h
-        fd = (G.loss_and_grad(wp, *args, bf=False)[0] - G.loss_and_grad(wm, *args, bf=False)[0]) / (2 * h)
+        fd = (G.loss_and_grad(wp, *args, bf=False, **kw)[0] - G.loss_and_grad(wm, *args, bf=False, **kw)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
 
 

@@ -28,7 +28,12 @@ def _ref_for(case, tmp_path, mesh=None):
     return pyoracle.RefProblem(d, fam, int(enum), int(scheme), ic, params)
 
 
-@pytest.mark.parametrize("case", cases.CASES, ids=cases.IDS)
+# the reference builds no 3D mesh with a 7-point stencil (impl/mesh_ccu.hpp:367-369),
+# so 3D WENO5 has no reference run to compare with
+REF_CASES = [c for c in cases.CASES if not (c[0].startswith("euler3d") and c[0].endswith("Weno5"))]
+
+
+@pytest.mark.parametrize("case", REF_CASES, ids=[c[0] for c in REF_CASES])
 def test_oracle_bit_exact_with_reference(case, oracle, tmp_path):
     mesh, prob = cases.make(case)
     ref = _ref_for(case, tmp_path, mesh)
@@ -43,7 +48,7 @@ def test_oracle_bit_exact_with_reference(case, oracle, tmp_path):
         assert np.array_equal(Jo, Jr)
 
 
-@pytest.mark.parametrize("case", cases.CASES, ids=cases.IDS)
+@pytest.mark.parametrize("case", REF_CASES, ids=[c[0] for c in REF_CASES])
 def test_product_setup_matches_reference(case, tmp_path):
     """Jacobian sparsity (bit-exact), sizes and initial condition."""
     mesh, prob = cases.make(case)
@@ -61,7 +66,7 @@ def test_product_setup_matches_reference(case, tmp_path):
     assert np.abs(U0 - Ur).max() <= 1e-14 * max(1.0, np.abs(Ur).max())
 
 
-SAMPLE_CASES = [c for c in cases.CASES if c[0] in (
+SAMPLE_CASES = [c for c in REF_CASES if c[0] in (
     "euler2d-riemann-FirstOrder", "euler2d-riemann-Weno5", "swe-slipwall-Weno3",
     "burgers-periodic-Weno5", "euler1d-sod-Weno5", "euler3d-sedov-Weno3")]
 
