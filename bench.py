@@ -42,6 +42,41 @@ def traffic_bytes(n):
         return None
 
 
+def extra_workloads():
+    """The batched shallow-water workload (384 slip-wall instances, RK4 x 500)
+    and the implicit Burgers workload (512^2 BDF2 x 400 steps, Newton to
+    ||R|| <= 1e-10 with the previous-step, extrapolated and learned guesses).
+    Reported beside the headline line; failures are reported, not fatal."""
+    out = {}
+    try:
+        from pressiodemoapps_amd import batched
+        t0 = time.perf_counter()
+        batch, _, sizes = batched.build(384, seed=7)
+        setup = time.perf_counter() - t0
+        r = batched.run(batch, sizes, steps=500, dt=0.01, log=log)
+        r["setup_s"] = setup
+        out["batched_swe"] = r
+        log(f"[bench] batched SWE: {r['cell_steps_per_s']:.3e} cell-steps/s, {r['wall_s']:.2f} s for 500 RK4 steps")
+        del batch
+    except Exception as e:  # reported, never fatal
+        out["batched_swe"] = {"error": str(e)}
+        log(f"[bench] batched SWE failed: {e}")
+    try:
+        from pressiodemoapps_amd import learned
+        if os.path.exists(learned.DEFAULT_WEIGHTS):
+            net, meta = learned.load(learned.DEFAULT_WEIGHTS, 512)
+            r = learned.compare(net, 512, 400, seed=77, log=log)
+            r["training"] = meta
+            out["implicit_burgers"] = r
+            log(f"[bench] implicit Burgers: learned/previous-step speedup {r['speedup_learned_vs_previous_step']:.2f}")
+        else:
+            out["implicit_burgers"] = {"error": "no trained weights at " + learned.DEFAULT_WEIGHTS}
+    except Exception as e:  # reported, never fatal
+        out["implicit_burgers"] = {"error": str(e)}
+        log(f"[bench] implicit Burgers failed: {e}")
+    return out
+
+
 def cpu_baseline(n_cpu, budget_s):
     """Reference CPU path (oracle/_ref, OpenMP) or the C restatement, timed on
     this host on an n_cpu x n_cpu sample of the same workload."""
@@ -99,6 +134,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched shallow-water and implicit Burgers measurements")
     ap.add_argument("--copy-calib", action="store_true",
                     help="after timing, run one plain 256 MiB device copy (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -179,6 +216,10 @@ def main():
         torch.cuda.synchronize()
         log("[bench] copy calibration: 256 MiB read + 256 MiB written")
 
+    extra = {}
+    if not args.no_extra:
+        extra = extra_workloads()
+
     cpu = None
     if not args.no_cpu:
         log("[bench] cpu baseline")
@@ -212,6 +253,7 @@ def main():
         "velocity_us": t_v * 1e6,
         "cpu_baseline": cpu,
     }
+    out.update(extra)
     print(json.dumps(out))
 
 

@@ -1203,7 +1203,59 @@ static void sources(ctx_t *c, int64_t r, const double *U, double t, double *Vr, 
         jadd(c, row + k, uB + k, dY);
       }
     }
+  } else if (pb->family == ORC_ADR2D) {
+    /* ADR problem A, impl/advection_diffusion_reaction_2d_prob_class.hpp:
+       inner rows add both axes' diffusion here (:1044-1049 with
+       addDiffusionToVelocityAndOptionalJacobianInnerCells :1057-1083),
+       near-boundary rows added theirs after each axis (adr_diffusion_near);
+       then the source f = 1 (advection_diffusion_reaction2d.hpp:87-99) and
+       the reaction -sigma*u (:1051-1052, Jacobian :509-511 and :840-872) */
+    const double two = 2;
+    const double dX = pb->p[2] * (c->hInv[0] * c->hInv[0]);
+    const double dY = pb->p[2] * (c->hInv[1] * c->hInv[1]);
+    const double sigma = pb->p[3];
+    const int64_t row = r;
+    const int32_t ui = G(M, r, 0), uL = G(M, r, 1), uF = G(M, r, 2), uR = G(M, r, 3), uB = G(M, r, 4);
+    const int near = c->ghostRow[r] >= 0;
+    if (!near) {
+      Vr[0] += dX * (U[uR] - two * U[ui] + U[uL]);
+      Vr[0] += dY * (U[uF] - two * U[ui] + U[uB]);
+      if (withJac) {
+        jadd(c, row, ui, -two * dX - two * dY);
+        jadd(c, row, uL, dX);
+        jadd(c, row, uF, dY);
+        jadd(c, row, uR, dX);
+        jadd(c, row, uB, dY);
+      }
+    }
+    Vr[0] += 1.0;
+    Vr[0] -= sigma * U[ui];
+    if (withJac) {
+      if (!near) {
+        jadd(c, row, ui, -sigma);
+      } else {
+        /* the missing neighbour is the ghost -u of the cell itself */
+        double self = -two * dX - two * dY - sigma;
+        if (uL != -1) jadd(c, row, uL, dX); else self += -dX;
+        if (uF != -1) jadd(c, row, uF, dY); else self += -dY;
+        if (uR != -1) jadd(c, row, uR, dX); else self += -dX;
+        if (uB != -1) jadd(c, row, uB, dY); else self += -dY;
+        jadd(c, row, ui, self);
+      }
+    }
   }
+}
+
+/* ADR diffusion of a near-boundary row along axis a from its stencil, the
+   missing neighbour read from the ghost layer,
+   impl/advection_diffusion_reaction_2d_prob_class.hpp:946-976 and :658-676 */
+static void adr_diffusion_near(ctx_t *c, int64_t r, int a, const double *U, double *Vr)
+{
+  double st[3];
+  gather(c, r, a, 1, U, st);
+  const double two = 2;
+  const double d = c->pb->p[2] * (c->hInv[a] * c->hInv[a]);
+  Vr[0] += d * (st[2] - two * st[1] + st[0]);
 }
 
 static void eval_row(ctx_t *c, int64_t r, const double *U, double t, double *V, int withJac)
@@ -1246,6 +1298,7 @@ static void eval_row(ctx_t *c, int64_t r, const double *U, double t, double *V, 
       if (near) jac_boundary(c, r, a, JLn, JLp, JRn, JRp);
       else jac_interior(c, r, a, &f, JLn, JLp, JRn, JRp);
     }
+    if (pb->family == ORC_ADR2D && near) adr_diffusion_near(c, r, a, U, Vr);
   }
   sources(c, r, U, t, Vr, withJac);
   for (int k = 0; k < m; ++k) V[r * m + k] = Vr[k];

@@ -17,6 +17,15 @@ namespace pda {
 namespace {
 using namespace fam;
 
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2):
+// workgroup b runs on XCD b % 8.  Renumbering b -> the b/8-th item of XCD
+// b%8's contiguous share of [0, n) gives each XCD a contiguous range of
+// strips, so state rows shared by neighbouring strips are read once per L2.
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t n) {
+  const int64_t q = n / 8, r = n % 8, x = b % 8, k = b / 8;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // ------------------------------------------------ velocity 2D, row marching
 // A wavefront owns a segment of up to 63 consecutive box cells of a grid row
 // and marches up a strip of rows.  Lane l evaluates the x face on the left
@@ -135,7 +144,7 @@ template <class Ph, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march2d(View P, Box B, int rows_per_strip,
                                                      const double *__restrict__ U,
                                                      double *__restrict__ V) {
-  vel_march_body<Ph, W>(P, B, rows_per_strip, int64_t(blockIdx.x), U, V);
+  vel_march_body<Ph, W>(P, B, rows_per_strip, xcd_contiguous(int64_t(blockIdx.x), int64_t(gridDim.x)), U, V);
 }
 
 // many independent instances (same family and scheme, each with its own
@@ -456,6 +465,8 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler2d(View P, Box B, int rows_p
     rows::jacobian_row<PhEuler2, 1, false>(P, int64_t(blockIdx.x) * 64 + threadIdx.x, U, nullptr, J);
     return;
   }
+  // (the XCD-contiguous renumbering of the velocity kernel measured 9 %
+  // slower here: 171.6 vs 157.9 us per call at 1024 x 1024)
   jac_fo_euler2d_seg(P, B, rows_per_strip, int64_t(blockIdx.x) - nframe, U, J);
 }
 

@@ -80,6 +80,22 @@ struct PhBurgers : MatSided<PhBurgers, 2> {
   }
 };
 
+// advection-diffusion-reaction problem A: one scalar advected with the
+// constant velocity (p[0], p[1]); the face flux takes the minus-side state
+// only (impl/advection_diffusion_reaction_2d_flux_mixin.hpp:95-105, 210-222)
+struct PhAdr : MatSided<PhAdr, 1> {
+  static constexpr int M = 1, DIM = 2;
+  template <int A>
+  __device__ static void flux(const View &P, const double *l, const double *, double *F) {
+    F[0] = l[0] * P.p[A];
+  }
+  template <int A>
+  __device__ static void fjac(const View &P, const double *, const double *, double *JL, double *JR) {
+    JL[0] = P.p[A];
+    JR[0] = 0.;
+  }
+};
+
 template <class Ph> struct Sources {
   // V sources after the axis sweep; the Jacobian part updates the diagonal
   // block held in registers (self) and returns the diagonal addend of
@@ -131,6 +147,72 @@ template <> struct Sources<PhBurgers> {
   __device__ static double offdiag(const View &P, bool near, int a, int k, int j) {
     if (near || k != j) return 0.;
     return P.p[0] * (P.hinv[a] * P.hinv[a]);
+  }
+};
+
+// ADR diffusion (D = p[2]) on inner rows, then the source f = 1 and the
+// reaction -sigma*u (sigma = p[3]),
+// impl/advection_diffusion_reaction_2d_prob_class.hpp:1040-1083 (velocity),
+// :490-512 (inner Jacobian) and :840-872 (near-boundary Jacobian: a missing
+// neighbour is the ghost -u, so its diffusion weight moves to the diagonal
+// with a minus sign); the near-boundary diffusion itself is added per axis
+// by AxisTerms<PhAdr>
+template <> struct Sources<PhAdr> {
+  __device__ static void apply(const View &P, int64_t r, bool near, const double *U, int32_t self,
+                               double *v, double *J) {
+#pragma clang fp contract(off)
+    const int32_t *gr = P.graph + r * P.ncols;
+    const double dX = P.p[2] * (P.hinv[0] * P.hinv[0]);
+    const double dY = P.p[2] * (P.hinv[1] * P.hinv[1]);
+    const double sigma = P.p[3];
+    const double u = U[self];
+    if (!near) {
+      v[0] += dX * (U[gr[3]] - 2. * u + U[gr[1]]);
+      v[0] += dY * (U[gr[2]] - 2. * u + U[gr[4]]);
+    }
+    v[0] += 1.;
+    v[0] -= sigma * u;
+    if (J) {
+      if (!near) {
+        J[0] += -2. * dX - 2. * dY;
+        J[0] -= sigma;
+      } else {
+        double sv = -2. * dX - 2. * dY - sigma;
+        if (gr[1] == -1) sv += -dX;
+        if (gr[2] == -1) sv += -dY;
+        if (gr[3] == -1) sv += -dX;
+        if (gr[4] == -1) sv += -dY;
+        J[0] += sv;
+      }
+    }
+  }
+  // the diffusion weight of every present first neighbour, inner and near
+  __device__ static double offdiag(const View &P, bool, int a, int, int) {
+    return P.p[2] * (P.hinv[a] * P.hinv[a]);
+  }
+};
+
+// Terms a family adds to a near-boundary row right after each axis sweep
+// (before the next axis), and whether the near-boundary Jacobian's
+// first-neighbour blocks take Sources<Ph>::offdiag.
+template <class Ph> struct AxisTerms {
+  static constexpr bool near_offdiag = false;
+  template <int A>
+  __device__ static void velocity(const View &, const int32_t *, const double *, double *) {}
+};
+
+// ADR near-boundary diffusion along axis A from the cell's first
+// neighbours, a missing one read as its ghost -u,
+// impl/advection_diffusion_reaction_2d_prob_class.hpp:946-976
+template <> struct AxisTerms<PhAdr> {
+  static constexpr bool near_offdiag = true;
+  template <int A>
+  __device__ static void velocity(const View &P, const int32_t *gr, const double *U, double *acc) {
+#pragma clang fp contract(off)
+    const int32_t l = gr[A == 0 ? 1 : 4], r = gr[A == 0 ? 3 : 2];
+    const double u = U[gr[0]];
+    const double ul = l >= 0 ? U[l] : -u, ur = r >= 0 ? U[r] : -u;
+    acc[0] += (P.p[2] * (P.hinv[A] * P.hinv[A])) * (ur - 2. * u + ul);
   }
 };
 

@@ -101,6 +101,7 @@ static int vel_generic(const View &v, const pda_problem_s &p, const double *U, d
     case PDA_EULER3D: return vel_dispatch<PhEuler3>(v, p.w, U, V, s);
     case PDA_SWE2D: return vel_dispatch<PhSwe>(v, p.w, U, V, s);
     case PDA_BURGERS2D: return vel_dispatch<PhBurgers>(v, p.w, U, V, s);
+    case PDA_ADR2D: return vel_dispatch<PhAdr>(v, p.w, U, V, s);
     default: set_error("velocity: unsupported family"); return PDA_E_UNSUPPORTED;
   }
 }
@@ -113,6 +114,7 @@ static int jac_generic(const View &v, const pda_problem_s &p, const double *U, d
     case PDA_EULER3D: return jac_dispatch<PhEuler3>(v, p.w, U, V, J, s);
     case PDA_SWE2D: return jac_dispatch<PhSwe>(v, p.w, U, V, J, s);
     case PDA_BURGERS2D: return jac_dispatch<PhBurgers>(v, p.w, U, V, J, s);
+    case PDA_ADR2D: return jac_dispatch<PhAdr>(v, p.w, U, V, J, s);
     default: set_error("jacobian: unsupported family"); return PDA_E_UNSUPPORTED;
   }
 }

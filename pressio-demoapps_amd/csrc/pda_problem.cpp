@@ -106,6 +106,9 @@ int ghost_rule_of(int family, int prob, bool *supported) {
     case PDA_BURGERS2D:
       if (prob == 0) return GHOST_NONE;             // BurgersPeriodic
       break;
+    case PDA_ADR2D:
+      if (prob == 0) return GHOST_OPPOSITE;         // ProblemA, homogeneous Dirichlet
+      break;
     default:
       break;
   }
@@ -255,6 +258,8 @@ int initial_condition_host(const pda_problem_s &p, std::vector<double> &U) {
       }
       return PDA_OK;
     }
+    case PDA_ADR2D: // zero state, impl/advection_diffusion_reaction_2d_initial_condition.hpp:55-60
+      return PDA_OK;
   }
   return PDA_E_UNSUPPORTED;
 }
@@ -262,9 +267,11 @@ int initial_condition_host(const pda_problem_s &p, std::vector<double> &U) {
 // Jacobian BC factors of near-boundary rows (first-order ghost dependence):
 // Euler 1D impl/euler_1d_prob_class.hpp:608-610, Euler 2D
 // impl/euler_2d_prob_class.hpp:1115-1140, Euler 3D
-// impl/euler_3d_prob_class.hpp:1017-1044, SWE impl/swe_2d_prob_class.hpp:839-854
+// impl/euler_3d_prob_class.hpp:1017-1044, SWE impl/swe_2d_prob_class.hpp:839-854,
+// ADR impl/advection_diffusion_reaction_2d_prob_class.hpp:640-641 and :809-812
 void bc_factors(const pda_problem_s &p, int64_t r, int a, double *f) {
   for (int k = 0; k < 5; ++k) f[k] = 1.;
+  if (p.family == PDA_ADR2D) f[0] = -1.;
   if (p.family == PDA_SWE2D) f[a == 0 ? 1 : 2] = -1.;
   if (p.family == PDA_EULER3D && p.prob == 1 && mesh_has_bd(*p.mesh, r, a, 0)) f[1 + a] = -1.;
 }
@@ -406,6 +413,11 @@ int build(pda_problem_s &p) {
     }
   if (p.family == PDA_EULER3D && p.prob == 1)
     for (int a = 0; a < 3; ++a) v.gsign[a][0][1 + a] = -1.;
+  // homogeneous Dirichlet: every ghost is the negated source cell
+  // (impl/advection_diffusion_reaction_2d_ghost_filler_problemA.hpp:97-133)
+  if (p.family == PDA_ADR2D)
+    for (int a = 0; a < 3; ++a)
+      for (int s = 0; s < 2; ++s) v.gsign[a][s][0] = -1.;
   return PDA_OK;
 }
 

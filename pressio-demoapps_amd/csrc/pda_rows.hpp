@@ -99,8 +99,15 @@ __device__ __forceinline__ void velocity_row(const View &P, int64_t t, const dou
 #pragma unroll
   for (int k = 0; k < M; ++k) acc[k] = 0.;
   velocity_axis<Ph, W, 0>(P, gr, g, U, acc);
-  if constexpr (Ph::DIM > 1) velocity_axis<Ph, W, 1>(P, gr, g, U, acc);
-  if constexpr (Ph::DIM > 2) velocity_axis<Ph, W, 2>(P, gr, g, U, acc);
+  if (g >= 0) AxisTerms<Ph>::template velocity<0>(P, gr, U, acc);
+  if constexpr (Ph::DIM > 1) {
+    velocity_axis<Ph, W, 1>(P, gr, g, U, acc);
+    if (g >= 0) AxisTerms<Ph>::template velocity<1>(P, gr, U, acc);
+  }
+  if constexpr (Ph::DIM > 2) {
+    velocity_axis<Ph, W, 2>(P, gr, g, U, acc);
+    if (g >= 0) AxisTerms<Ph>::template velocity<2>(P, gr, U, acc);
+  }
   Sources<Ph>::apply(P, r, g >= 0, U, gr[0], acc, nullptr);
 #pragma unroll
   for (int k = 0; k < M; ++k) V[r * M + k] = acc[k];
@@ -231,7 +238,10 @@ __device__ __forceinline__ void jac_axis_near(const View &P, const int32_t *gr, 
 #pragma unroll
     for (int k = 0; k < M; ++k)
 #pragma unroll
-      for (int j = 0; j < M; ++j) b[k][j] = 0. + JLn[k * M + j] * h;
+      for (int j = 0; j < M; ++j) {
+        b[k][j] = 0. + JLn[k * M + j] * h;
+        if constexpr (AxisTerms<Ph>::near_offdiag) b[k][j] += Sources<Ph>::offdiag(P, true, A, k, j);
+      }
     put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 0, 0)], b);
   }
   if (r0 != -1) {
@@ -239,7 +249,10 @@ __device__ __forceinline__ void jac_axis_near(const View &P, const int32_t *gr, 
 #pragma unroll
     for (int k = 0; k < M; ++k)
 #pragma unroll
-      for (int j = 0; j < M; ++j) b[k][j] = 0. + (-JRp[k * M + j]) * h;
+      for (int j = 0; j < M; ++j) {
+        b[k][j] = 0. + (-JRp[k * M + j]) * h;
+        if constexpr (AxisTerms<Ph>::near_offdiag) b[k][j] += Sources<Ph>::offdiag(P, true, A, k, j);
+      }
     put_block<M>(P, J, base, rs, sl[colof<Ph::DIM>(A, 1, 0)], b);
   }
   if (l0 == -1) {
@@ -284,8 +297,15 @@ __device__ __forceinline__ void jacobian_row(const View &P, int64_t t, const dou
   } else {
     fill_row_ghosts<Ph::DIM, M>(P, W, g, gr, U);
     jac_axis_near<Ph, W, 0, WV>(P, gr, g, sl, U, acc, self, J, base, rs);
-    if constexpr (Ph::DIM > 1) jac_axis_near<Ph, W, 1, WV>(P, gr, g, sl, U, acc, self, J, base, rs);
-    if constexpr (Ph::DIM > 2) jac_axis_near<Ph, W, 2, WV>(P, gr, g, sl, U, acc, self, J, base, rs);
+    if constexpr (WV) AxisTerms<Ph>::template velocity<0>(P, gr, U, acc);
+    if constexpr (Ph::DIM > 1) {
+      jac_axis_near<Ph, W, 1, WV>(P, gr, g, sl, U, acc, self, J, base, rs);
+      if constexpr (WV) AxisTerms<Ph>::template velocity<1>(P, gr, U, acc);
+    }
+    if constexpr (Ph::DIM > 2) {
+      jac_axis_near<Ph, W, 2, WV>(P, gr, g, sl, U, acc, self, J, base, rs);
+      if constexpr (WV) AxisTerms<Ph>::template velocity<2>(P, gr, U, acc);
+    }
   }
   Sources<Ph>::apply(P, r, g >= 0, U, gr[0], acc, &self[0][0]);
   put_block<M>(P, J, base, rs, sl[0], self);

@@ -25,6 +25,17 @@ for s in (S.FirstOrder, S.Weno3):
         (f"euler3d-periodic-{s.name}", pda.Euler3d.PeriodicSmooth, s, (8, 8, 8), (-1, 1, -1, 1, -1, 1), 5, True, 0, None),
     ]
 
+# advection-diffusion-reaction problem A (homogeneous Dirichlet walls, zero
+# initial state), default parameters and a stronger-diffusion set
+ADR_PARAMS = {"ux": 0.35, "uy": -0.6, "diffusion": 0.02, "sigma": 0.7}
+for s in (S.FirstOrder, S.Weno3, S.Weno5):
+    CASES += [
+        (f"adr-probA-{s.name}", pda.AdvectionDiffusionReaction2d.ProblemA, s, (18, 16), (0, 1, 0, 1), 7, (), 0,
+         None),
+        (f"adr-custom-{s.name}", pda.AdvectionDiffusionReaction2d.ProblemA, s, (15, 19), (0, 1, 0, 1), 7, (), 0,
+         dict(ADR_PARAMS)),
+    ]
+
 # 3D WENO5 (build-defined: the reference stops at WENO3 in 3D); pinned by
 # conservation and axis-permutation equivariance in test_oracle_kat.py
 CASES += [
@@ -54,6 +65,8 @@ ORACLE_PARAMS = {
     3: lambda p: [1.4],
     4: lambda p: [(p or {}).get("gravity", 9.8), (p or {}).get("coriolis", -3.0)],
     5: lambda p: [(p or {}).get("diffusion", 1e-5)],
+    6: lambda p: [(p or {}).get("ux", 0.5 * np.cos(np.pi / 3)), (p or {}).get("uy", 0.5 * np.sin(np.pi / 3)),
+                  (p or {}).get("diffusion", 0.001), (p or {}).get("sigma", 1.0)],
 }
 
 
@@ -78,6 +91,9 @@ def perturbed_state(prob, seed, amp=0.01):
     rc = pda.lib().pda_initial_condition_host(prob.handle, U0.ctypes.data_as(ctypes.c_void_p))
     assert rc == 0
     rng = np.random.default_rng(seed)
+    if not U0.any():  # zero initial state (advection-diffusion-reaction): a smooth bump plus noise
+        n = U0.size
+        return np.sin(np.linspace(0.0, 3.0, n)) ** 2 + amp * rng.uniform(-1.0, 1.0, n)
     return U0 * (1.0 + amp * rng.uniform(-1.0, 1.0, U0.shape))
 
 

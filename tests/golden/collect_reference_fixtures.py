@@ -59,6 +59,13 @@ TRAJ = [
     "tests_cpp/eigen_1d_euler_sod_implicit/weno3/gold.txt",
     "tests_cpp/eigen_2d_euler_riemann_implicit/firstorder/rho_gold.txt",
     "tests_cpp/eigen_2d_euler_riemann_implicit/firstorder/p_gold.txt",
+    # advection-diffusion-reaction problem A: RK4 (18x18) and Crank-Nicolson (17x17)
+    "tests_cpp/eigen_2d_advdiffreac_probA_explicit/firstorder/gold.txt",
+    "tests_cpp/eigen_2d_advdiffreac_probA_explicit/weno3/gold.txt",
+    "tests_cpp/eigen_2d_advdiffreac_probA_explicit/weno5/gold.txt",
+    "tests_cpp/eigen_2d_advdiffreac_probA_implicit/firstorder/gold.txt",
+    "tests_cpp/eigen_2d_advdiffreac_probA_implicit/weno3/gold.txt",
+    "tests_cpp/eigen_2d_advdiffreac_probA_implicit/weno5/gold.txt",
 ]
 
 

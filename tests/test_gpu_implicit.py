@@ -91,3 +91,22 @@ def test_burgers_bdf2_residual_rule(oracle, torch):
               f"||R|| dev {st['residual_norm']:.2e} oracle {np.linalg.norm(R):.2e}", flush=True)
         assert np.linalg.norm(R) <= 2e-10
         Uprev = y_n
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3"),
+                                            (S.Weno5, 7, "weno5")])
+def test_adr_crank_nicolson_matches_reference_gold(scheme, st, name, torch):
+    # tests_cpp/eigen_2d_advdiffreac_probA_implicit: 17x17 on [0,1]^2, zero
+    # initial state, CN dt 0.05 x 40, Newton tolerance 1e-5
+    print(f"[gpu-implicit] adr cn {name}", flush=True)
+    mesh = pda.create_structured_mesh((17, 17), (0.0, 1.0, 0.0, 1.0), st)
+    prob = pda.create_problem(mesh, pda.AdvectionDiffusionReaction2d.ProblemA, scheme)
+    ip = pda.ImplicitStepper(prob, pda.ImplicitScheme.CrankNicolson)
+    ip.set_options(pda.StopRule.Correction, 1e-5)
+    U = prob.initialCondition()
+    for k in range(40):
+        ip.step(U, k * 0.05, 0.05)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_advdiffreac_probA_implicit__{name}__gold.npy"))
+    Uh = U.cpu().numpy()
+    print(f"[gpu-implicit] adr {name}: max |U-gold| {np.abs(Uh - gold).max():.2e}", flush=True)
+    assert np.allclose(Uh, gold, rtol=1e-10, atol=1e-12)

@@ -118,3 +118,19 @@ def test_rk2_and_observer(oracle, torch):
         y = y + k2 * 0.5 + k1 * 0.5
     torch.cuda.synchronize()
     assert cases.rel_err(U.cpu().numpy(), y) <= 1e-12
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3"),
+                                            (S.Weno5, 7, "weno5")])
+def test_adr_rk4_matches_reference_gold(scheme, st, name, torch):
+    # tests_cpp/eigen_2d_advdiffreac_probA_explicit: 18x18 on [0,1]^2, zero
+    # initial state, default parameters, RK4 dt 0.01 x 500 (graph-replayed)
+    import os
+    print(f"[gpu-step] adr rk4 {name}", flush=True)
+    mesh = pda.create_structured_mesh((18, 18), (0.0, 1.0, 0.0, 1.0), st)
+    prob = pda.create_problem(mesh, pda.AdvectionDiffusionReaction2d.ProblemA, scheme)
+    U = prob.initialCondition()
+    pda.advanceRK4(prob, U, 0.01, 500)
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref", "traj",
+                                f"eigen_2d_advdiffreac_probA_explicit__{name}__gold.npy"))
+    assert np.allclose(U.cpu().numpy(), gold, rtol=1e-10, atol=1e-12)

@@ -170,3 +170,36 @@ def test_riemann2d_cn_implicit():
     base = "eigen_2d_euler_riemann_implicit__firstorder__"
     assert np.allclose(rho, np.load(os.path.join(TRAJ, base + "rho_gold.npy")), rtol=1e-10, atol=1e-12)
     assert np.allclose(p, np.load(os.path.join(TRAJ, base + "p_gold.npy")), rtol=1e-10, atol=1e-12)
+
+
+ADR_SCHEMES = [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3"), (S.Weno5, 7, "weno5")]
+ADR_DEFAULT = (0.5 * np.cos(np.pi / 3), 0.5 * np.sin(np.pi / 3), 0.001, 1.0)
+
+
+@pytest.mark.parametrize("scheme,st,name", ADR_SCHEMES)
+def test_adr_probA_rk4(scheme, st, name):
+    # tests_cpp/eigen_2d_advdiffreac_probA_explicit: 18x18 on [0,1]^2, zero
+    # initial state, default parameters, RK4 dt 0.01 x 500
+    U0, f = _setup(pda.AdvectionDiffusionReaction2d.ProblemA, scheme, (18, 18), (0.0, 1.0, 0.0, 1.0), st,
+                   oparams=ADR_DEFAULT)
+    assert not U0.any()
+    U = rk4(f, U0, 0.01, 500)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_advdiffreac_probA_explicit__{name}__gold.npy"))
+    assert np.allclose(U, gold, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme,st,name", ADR_SCHEMES)
+def test_adr_probA_cn_implicit(scheme, st, name):
+    # tests_cpp/eigen_2d_advdiffreac_probA_implicit: 17x17 on [0,1]^2,
+    # Crank-Nicolson dt 0.05 x 40, Newton tolerance 1e-5
+    mesh = pda.create_structured_mesh((17, 17), (0.0, 1.0, 0.0, 1.0), st)
+    orc = pyoracle.Oracle()
+    ms = orc.from_mesh(mesh)
+    pb = orc.problem(6, 0, int(scheme), list(ADR_DEFAULT))
+    fj = lambda y: orc.velocity_and_jacobian(ms, pb, y)  # noqa: E731
+    U = np.zeros(mesh.stencilMeshSize())
+    for _ in range(40):
+        fn = orc.velocity(ms, pb, U)
+        U = newton_implicit("cn", U, None, 0.05, fj, 1e-5, f_n=fn)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_advdiffreac_probA_implicit__{name}__gold.npy"))
+    assert np.allclose(U, gold, rtol=1e-10, atol=1e-12)
