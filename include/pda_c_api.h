@@ -236,10 +236,12 @@ void pda_implicit_destroy(pda_implicit ip);
 
 /* A 4-layer 3x3 convolutional network (32 channels, BF16 weights and
    activations, FP32 accumulation on MFMA) predicting the next state of a
-   periodic nx x ny two-component field from (y_n, y_{n-1}, nu):
-     guess = 2 y_n - y_{n-1} + s_o * net(y_n, (y_n - y_{n-1})/s_d,
-                                          dt f(y_n)/s_f, (nu - nu_mean)/nu_scale)
-   where f(y_n) is the problem's velocity at y_n (d_fn, nullable = 0).
+   periodic nx x ny two-component field stepped with BDF2:
+     base  = (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1})   (FP64)
+     guess = base + net(y_n, dt f_n/s_f, dt (f_n - f_{n-1})/s_g,
+                        (nu - nu_mean)/nu_scale) * (2/3) dt (f_n - f_{n-1})
+   where f_n = f(y_n), f_{n-1} = f(y_{n-1}) are the problem's velocity at the
+   two previous states (d_fn, d_fnm1).
    No counterpart in the reference: it feeds pda_implicit_step's d_guess,
    and Newton on the exact Jacobian still decides the converged state.
    States are device arrays of nx*ny*2 doubles in cell order j*nx + i.
@@ -253,21 +255,24 @@ int pda_guessnet_num_params(pda_guessnet net, int64_t *n);
    resets the optimizer state. */
 int pda_guessnet_set_params(pda_guessnet net, const float *w, int64_t n);
 int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n);
-int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale,
-                            double dt_over_sf);
+/* dt: the BDF2 time step; s_f, s_g, s_o: spreads of dt f_n, dt (f_n - f_{n-1})
+   and y_{n+1} - base; all positive */
+int pda_guessnet_set_scales(pda_guessnet net, double dt, double s_f, double s_g, double s_o,
+                            double nu_mean, double nu_scale);
 int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1,
-                         const double *d_fn, double nu, double *d_guess, pda_stream stream);
-/* loss = mean((net - (y_{n+1} - 2 y_n + y_{n-1})/s_o)^2) and its gradient
-   w.r.t. the parameters (host arrays, nullable; out = the net's raw
-   output, nx*ny*2 floats). */
+                         const double *d_fn, const double *d_fnm1, double nu, double *d_guess,
+                         pda_stream stream);
+/* loss = mean(((net * gate - (y_{n+1} - base))/s_o)^2), gate =
+   (2/3) dt (f_n - f_{n-1}), and its gradient w.r.t. the parameters (host
+   arrays, nullable; out = the net's raw output, nx*ny*2 floats). */
 int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1,
-                           const double *d_fn, const double *d_unp1, double nu, float *grad,
-                           float *loss, float *out, pda_stream stream);
+                           const double *d_fn, const double *d_fnm1, const double *d_unp1, double nu,
+                           float *grad, float *loss, float *out, pda_stream stream);
 /* one Adam step (beta 0.9 / 0.999) on that loss; loss (nullable) is
    reported before the update */
 int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1,
-                            const double *d_fn, const double *d_unp1, double nu, float lr,
-                            float *loss, pda_stream stream);
+                            const double *d_fn, const double *d_fnm1, const double *d_unp1, double nu,
+                            float lr, float *loss, pda_stream stream);
 
 /* Block view of the Jacobian (m x m blocks).  For each block-row r and block
    slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values

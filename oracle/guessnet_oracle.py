@@ -5,7 +5,10 @@ The network has no counterpart in the reference; this file states its
 definition directly: periodic 3x3 convolutions on nx*ny two-component
 fields, layers (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 2), SiLU
 after the first three, flat FP32 parameters per layer W[tap][cout][cin] then
-bias[cout], tap = 3*(dy+1) + (dx+1) reading pixel (y+dy, x+dx).
+bias[cout], tap = 3*(dy+1) + (dx+1) reading pixel (y+dy, x+dx).  The guess
+is base + output * gate with the FP64 BDF2 base
+(4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1}) and the gate
+(2/3) dt (f_n - f_{n-1}).
 
 `bf16=True` rounds to BF16 (round to nearest even) where the device stores
 BF16: features, weights, hidden activations and the backward's dZ; all
@@ -59,17 +62,25 @@ def _dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def features(un, unm1, nu, H, W, s_d, nu_mean, nu_scale, rb=True, fn=None, dt_over_sf=0.0):
-    """Channels: u, v, du/s_d, dv/s_d, dt f_u/s_f, dt f_v/s_f, nu_hat, 0."""
-    u = un.reshape(H, W, 2)
-    d = (un - unm1).reshape(H, W, 2) / s_d
+def features(un, fn, fnm1, nu, H, W, dt, s_f, s_g, nu_mean, nu_scale, rb=True):
+    """Channels: u, v, dt f_u/s_f, dt f_v/s_f, dt (f_u - f_u,n-1)/s_g, the
+    same for v, nu_hat, 0."""
     X = np.zeros((H, W, 8))
-    X[..., 0:2] = u
-    X[..., 2:4] = d
-    if fn is not None:
-        X[..., 4:6] = fn.reshape(H, W, 2) * dt_over_sf
+    X[..., 0:2] = un.reshape(H, W, 2)
+    X[..., 2:4] = fn.reshape(H, W, 2) * (dt / s_f)
+    X[..., 4:6] = (fn - fnm1).reshape(H, W, 2) * (dt / s_g)
     X[..., 6] = (nu - nu_mean) / nu_scale
     return bf16(X) if rb else X
+
+
+def base(un, unm1, fn, fnm1, dt):
+    """BDF2 with f(y_{n+1}) extrapolated linearly in time."""
+    return (4.0 / 3.0) * un - (1.0 / 3.0) * unm1 + ((2.0 / 3.0) * dt) * (2.0 * fn - fnm1)
+
+
+def gate(fn, fnm1, dt):
+    """The base's extrapolation increment, which scales the output."""
+    return ((2.0 / 3.0) * dt) * (fn - fnm1)
 
 
 def forward(w, X0, bf=True):
@@ -91,17 +102,17 @@ def forward(w, X0, bf=True):
     return z, cache
 
 
-def loss_and_grad(w, un, unm1, unp1, nu, H, W, s_d, s_o, nu_mean, nu_scale, bf=True, fn=None,
-                  dt_over_sf=0.0):
-    X0 = features(un, unm1, nu, H, W, s_d, nu_mean, nu_scale, rb=bf, fn=fn, dt_over_sf=dt_over_sf)
+def loss_and_grad(w, un, unm1, fn, fnm1, unp1, nu, H, W, dt, s_f, s_g, s_o, nu_mean, nu_scale, bf=True):
+    X0 = features(un, fn, fnm1, nu, H, W, dt, s_f, s_g, nu_mean, nu_scale, rb=bf)
     out, cache = forward(w, X0, bf)
-    t = ((unp1 - 2.0 * un + unm1) / s_o).reshape(H, W, 2)
+    t = ((unp1 - base(un, unm1, fn, fnm1, dt)) / s_o).reshape(H, W, 2)
+    gs = (gate(fn, fnm1, dt) / s_o).reshape(H, W, 2)
+    d = out * gs - t
     if bf:
-        t = t.astype(np.float32).astype(np.float64)
-    d = out - t
+        d = d.astype(np.float32).astype(np.float64)
     npix = H * W
     loss = float(np.sum(d * d) / (npix * 2))
-    dz = 2.0 * d / (npix * 2)
+    dz = 2.0 * d * gs / (npix * 2)
     if bf:
         dz = bf16(dz)
     grads = []

@@ -36,13 +36,13 @@ __device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t n) {
 // strip) and no workgroup barrier is needed.
 constexpr int VSEG = 63;
 
-// state of cell (i, j) with the index along axis A possibly outside [0, n):
-// periodic wrap, or the mirror cell times the side's ghost sign (the
+// state of cell (i, j, k) with the index along axis A possibly outside
+// [0, n): periodic wrap, or the mirror cell times the side's ghost sign (the
 // GHOST_MIRROR rule, whose ghost values do not depend on the reading row)
 template <int M, int A>
 __device__ __forceinline__ void load_cell_mapped(const View &P, const Box &B, const double *__restrict__ U,
-                                                 int64_t i, int64_t j, double *q) {
-  int64_t c = A == 0 ? i : j;
+                                                 int64_t i, int64_t j, int64_t k, double *q) {
+  int64_t c = A == 0 ? i : (A == 1 ? j : k);
   const int64_t n = B.n[A];
   int side = -1;
   if (c < 0) {
@@ -52,7 +52,8 @@ __device__ __forceinline__ void load_cell_mapped(const View &P, const Box &B, co
     if (B.mode[A] == 1) c -= n;
     else { c = 2 * n - 1 - c; side = 1; }
   }
-  const int64_t cell = A == 0 ? j * B.n[0] + c : c * B.n[0] + i;
+  const int64_t ii = A == 0 ? c : i, jj = A == 1 ? c : j, kk = A == 2 ? c : k;
+  const int64_t cell = (kk * B.n[1] + jj) * B.n[0] + ii;
 #pragma unroll
   for (int k = 0; k < M; ++k) {
     const double v = U[cell * M + k];
@@ -60,15 +61,16 @@ __device__ __forceinline__ void load_cell_mapped(const View &P, const Box &B, co
   }
 }
 
-// flux of the face on the low side (along A) of cell (i, j)
+// flux of the face on the low side (along A) of cell (i, j, k)
 template <class Ph, int W, int A>
 __device__ __forceinline__ void face_flux_fast(const View &P, const Box &B, const double *__restrict__ U,
-                                               int64_t i, int64_t j, double *F) {
+                                               int64_t i, int64_t j, int64_t k, double *F) {
   constexpr int M = Ph::M;
   double st[2 * W][M];
 #pragma unroll
   for (int p = 0; p < 2 * W; ++p)
-    load_cell_mapped<M, A>(P, B, U, A == 0 ? i + p - W : i, A == 1 ? j + p - W : j, st[p]);
+    load_cell_mapped<M, A>(P, B, U, A == 0 ? i + p - W : i, A == 1 ? j + p - W : j, A == 2 ? k + p - W : k,
+                           st[p]);
   double qm[M], qp[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) {
@@ -92,10 +94,14 @@ __device__ __forceinline__ void vel_march_body(const View &P, const Box &B, int 
                                                const double *__restrict__ U, double *__restrict__ V) {
   constexpr int M = Ph::M;
   const int lane = threadIdx.x;
-  const int64_t nx = B.n[0];
+  const int64_t nx = B.n[0], ny = B.n[1];
   const int64_t bw = B.hi[0] - B.lo[0];
   const int64_t nseg = (bw + VSEG - 1) / VSEG;
-  const int64_t seg = bid % nseg, strip = bid / nseg;
+  const int64_t nstrip = (B.hi[1] - B.lo[1] + rows_per_strip - 1) / rows_per_strip;
+  // workgroup -> (plane, strip, segment); one plane in 2D
+  const int64_t plane = bid / (nseg * nstrip), rem = bid - plane * nseg * nstrip;
+  const int64_t seg = rem % nseg, strip = rem / nseg;
+  const int64_t kz = B.lo[2] + plane;
   const int64_t iseg = B.lo[0] + seg * VSEG;
   const int ncell = int(bw - seg * VSEG < VSEG ? bw - seg * VSEG : VSEG);
   const int64_t j0 = B.lo[1] + strip * rows_per_strip;
@@ -109,28 +115,36 @@ __device__ __forceinline__ void vel_march_body(const View &P, const Box &B, int 
   const int64_t i = iseg + (lane <= ncell ? lane : ncell);
   const int64_t ic = i < nx ? i : nx - 1;
   const bool own = lane < ncell;
-  const double hx = P.hinv[0], hy = P.hinv[1];
+  const double hx = P.hinv[0], hy = P.hinv[1], hz = P.hinv[2];
   // one call site per face kind, so that every face value is the same
   // whatever the strip height (the first pass only evaluates the face below
   // the strip)
   double fyb[M];
   for (int64_t j = j0 - 1; j < j1; ++j) {
     double fy[M];
-    face_flux_fast<Ph, W, 1>(P, B, U, ic, j + 1, fy);
+    face_flux_fast<Ph, W, 1>(P, B, U, ic, j + 1, kz, fy);
     if (j < j0) {
 #pragma unroll
       for (int k = 0; k < M; ++k) fyb[k] = fy[k];
       continue;
     }
-    const int64_t c = j * nx + ic;
+    const int64_t c = (kz * ny + j) * nx + ic;
     double fx[M], acc[M];
-    face_flux_fast<Ph, W, 0>(P, B, U, i, j, fx);
+    face_flux_fast<Ph, W, 0>(P, B, U, i, j, kz, fx);
 #pragma unroll
     for (int k = 0; k < M; ++k) {
       const double fr = __shfl_down(fx[k], 1, 64);
       acc[k] = 0. + hx * (fx[k] - fr);
       acc[k] += hy * (fyb[k] - fy[k]);
       fyb[k] = fy[k];
+    }
+    if constexpr (Ph::DIM == 3) {
+      // z faces below and above the cell (no carry across planes)
+      double fzb[M], fza[M];
+      face_flux_fast<Ph, W, 2>(P, B, U, ic, j, kz, fzb);
+      face_flux_fast<Ph, W, 2>(P, B, U, ic, j, kz + 1, fza);
+#pragma unroll
+      for (int k = 0; k < M; ++k) acc[k] += hz * (fzb[k] - fza[k]);
     }
     Sources<Ph>::apply(P, c, false, U, int32_t(c), acc, nullptr);
     if (own) {
@@ -141,7 +155,7 @@ __device__ __forceinline__ void vel_march_body(const View &P, const Box &B, int 
 }
 
 template <class Ph, int W>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march2d(View P, Box B, int rows_per_strip,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march(View P, Box B, int rows_per_strip,
                                                      const double *__restrict__ U,
                                                      double *__restrict__ V) {
   vel_march_body<Ph, W>(P, B, rows_per_strip, xcd_contiguous(int64_t(blockIdx.x), int64_t(gridDim.x)), U, V);
@@ -165,15 +179,6 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_
 // cells have the five blocks [back, left, self, right, front] in that
 // column order), so after the blocks are formed the wavefront writes the
 // segment as one contiguous run, 32 cells per LDS round.
-// LDS hand-off inside one wavefront (the workgroup is a single wave): LDS
-// instructions of a wave execute in order, so only the compiler must be kept
-// from reordering; unlike __syncthreads() this does not wait for the wave's
-// outstanding global stores.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 template <int M> struct Chunk { using T = double2; };
 template <> struct Chunk<1> { using T = double; };
@@ -476,19 +481,22 @@ bool has_box_kernels(const pda_problem_s &p) {
   return p.dim == 2 &&
          (p.family == PDA_EULER2D || p.family == PDA_SWE2D || p.family == PDA_BURGERS2D);
 }
+bool has_box_velocity(const pda_problem_s &p) {
+  return has_box_kernels(p) || (p.dim == 3 && p.family == PDA_EULER3D);
+}
 
 template <class Ph>
 static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
   const Box &B = p.vbox;
   // strips of rows_per_strip rows: ~8k one-wave groups for the 1024^2 grid
   const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
-  const int64_t bh = B.hi[1] - B.lo[1];
-  int64_t rps = (bh * nseg) / 8192;
+  const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
+  int64_t rps = (bh * nseg * planes) / 8192;
   rps = rps < 1 ? 1 : (rps > 64 ? 64 : rps);
-  const dim3 grid(unsigned(nseg * ((bh + rps - 1) / rps)));
-  if (p.w == 1) hipLaunchKernelGGL((k_vel_march2d<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
-  else if (p.w == 2) hipLaunchKernelGGL((k_vel_march2d<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
-  else hipLaunchKernelGGL((k_vel_march2d<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  const dim3 grid{unsigned(nseg * ((bh + rps - 1) / rps) * planes), 1, 1};
+  if (p.w == 1) hipLaunchKernelGGL((k_vel_march<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  else if (p.w == 2) hipLaunchKernelGGL((k_vel_march<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  else hipLaunchKernelGGL((k_vel_march<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
@@ -499,6 +507,7 @@ int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipS
     case PDA_EULER2D: return vel_box<PhEuler2>(p, U, V, s);
     case PDA_SWE2D: return vel_box<PhSwe>(p, U, V, s);
     case PDA_BURGERS2D: return vel_box<PhBurgers>(p, U, V, s);
+    case PDA_EULER3D: return vel_box<PhEuler3>(p, U, V, s);
   }
   return PDA_E_UNSUPPORTED;
 }
@@ -548,7 +557,7 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
 int64_t velocity_march_blocks(const Box &B, int rows_per_strip) {
   const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
   const int64_t bh = B.hi[1] - B.lo[1];
-  return nseg * ((bh + rows_per_strip - 1) / rows_per_strip);
+  return nseg * ((bh + rows_per_strip - 1) / rows_per_strip) * (B.hi[2] - B.lo[2]);
 }
 
 template <class Ph>

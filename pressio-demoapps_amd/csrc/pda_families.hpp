@@ -217,6 +217,16 @@ template <> struct AxisTerms<PhAdr> {
 };
 
 // ------------------------------------------------------------------ helpers
+// LDS hand-off inside one wavefront (the workgroup is a single wave): LDS
+// instructions of a wave execute in order, so only the compiler must be kept
+// from reordering; unlike __syncthreads() this does not wait for the wave's
+// outstanding global stores.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int DIM>
 __device__ __forceinline__ int colof(int a, int side, int L) {
   if (DIM == 1) return 1 + 2 * L + side;

@@ -5,13 +5,20 @@
 // on the device by the kernels below (forward, backward-data,
 // backward-weight, Adam).
 //
-//   features (8 ch): u_n, v_n, (u_n - u_{n-1})/s_d, (v_n - v_{n-1})/s_d,
-//                    dt f_u(y_n)/s_f, dt f_v(y_n)/s_f, nu_hat, 0
-//   (f = the problem's velocity at y_n, evaluated by the solver in FP64:
-//   the net receives the time derivative instead of differencing quantised
-//   states)
+//   features (8 ch): u_n, v_n, dt f_u(y_n)/s_f, dt f_v(y_n)/s_f,
+//                    dt (f_u(y_n) - f_u(y_{n-1}))/s_g, the same for v,
+//                    nu_hat, 0
+//   (f = the problem's velocity, evaluated by the solver in FP64: the net
+//   receives time derivatives instead of differencing quantised states)
 //   L1 8->32 SiLU, L2 32->32 SiLU, L3 32->32 SiLU, L4 32->2 (linear)
-//   guess  = 2 y_n - y_{n-1} + s_o * L4          (periodic padding)
+//   base   = (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f(y_n) - f(y_{n-1}))
+//            (BDF2 with f(y_{n+1}) extrapolated linearly; FP64)
+//   gate   = (2/3) dt (f(y_n) - f(y_{n-1}))  (the base's extrapolation step)
+//   guess  = base + L4 * gate                     (periodic padding)
+//   The network predicts the ratio of the base's remainder to the gate,
+//   O(1) where shocks move and near zero where the solution is smooth; the
+//   BF16 rounding of its output is scaled by the gate, so it vanishes
+//   where there is nothing to correct.
 //
 // Every convolution is an implicit GEMM: D[pixel][cout] += A[pixel][k] B[k][cout]
 // over K = taps x input channels, 32 K per MFMA.  An MFMA pairs A's and B's K
@@ -31,7 +38,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int C = 32;      // hidden channels
-constexpr int CIN0 = 8;    // feature channels (5 used)
+constexpr int CIN0 = 8;    // feature channels (7 used)
 constexpr int NOUT = 2;    // output channels
 constexpr int NL = 4;      // layers
 constexpr int MT = 4;      // 16-pixel M tiles per wavefront in the forward kernels
@@ -215,39 +222,55 @@ __global__ void k_bias_grad(const __bf16 *__restrict__ dZ, int64_t npix, int cou
   if (threadIdx.x == 0) db[c] = sh[0];
 }
 
-// features of the network input
-__global__ void k_features(const double *__restrict__ un, const double *__restrict__ unm1,
-                           const double *__restrict__ fn, double dt_over_sf, double nu_hat, double inv_sd,
-                           int64_t npix, __bf16 *__restrict__ X0) {
+// features of the network input: u, v, dt f(y_n) / s_f (2 channels),
+// dt (f(y_n) - f(y_{n-1})) / s_g (2 channels), nu_hat, 0
+__global__ void k_features(const double *__restrict__ un, const double *__restrict__ fn,
+                           const double *__restrict__ fnm1, double dt_over_sf, double dt_over_sg,
+                           double nu_hat, int64_t npix, __bf16 *__restrict__ X0) {
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (p >= npix) return;
-  const double u = un[2 * p], v = un[2 * p + 1];
-  const double du = (u - unm1[2 * p]) * inv_sd, dv = (v - unm1[2 * p + 1]) * inv_sd;
-  const double fu = fn ? fn[2 * p] * dt_over_sf : 0., fv = fn ? fn[2 * p + 1] * dt_over_sf : 0.;
   __bf16 *x = X0 + p * CIN0;
-  x[0] = __bf16(float(u));
-  x[1] = __bf16(float(v));
-  x[2] = __bf16(float(du));
-  x[3] = __bf16(float(dv));
-  x[4] = __bf16(float(fu));
-  x[5] = __bf16(float(fv));
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const double f = fn[2 * p + c];
+    x[c] = __bf16(float(un[2 * p + c]));
+    x[2 + c] = __bf16(float(f * dt_over_sf));
+    x[4 + c] = __bf16(float((f - fnm1[2 * p + c]) * dt_over_sg));
+  }
   x[6] = __bf16(float(nu_hat));
   x[7] = __bf16(0.f);
 }
 
-// guess = 2 y_n - y_{n-1} + s_o * out
-__global__ void k_guess(const double *__restrict__ un, const double *__restrict__ unm1,
-                        const float *__restrict__ out, double so, int64_t n, double *__restrict__ guess) {
-  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  guess[i] = 2. * un[i] - unm1[i] + so * double(out[i]);
+// FP64 base of the guess: the BDF2 relation with f(y_{n+1}) extrapolated
+// linearly in time, (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1})
+__device__ __forceinline__ double bdf2_base(const double *__restrict__ un, const double *__restrict__ unm1,
+                                            const double *__restrict__ fn, const double *__restrict__ fnm1,
+                                            double dt, int64_t i) {
+  return (4. / 3.) * un[i] - (1. / 3.) * unm1[i] + ((2. / 3.) * dt) * (2. * fn[i] - fnm1[i]);
 }
 
-// loss = mean((out - t)^2), t = (y_{n+1} - 2 y_n + y_{n-1}) / s_o; dZ of the
-// output layer (bf16, 32 channels, zero beyond NOUT); per-block loss sums
+// the gate: the base's own extrapolation increment (2/3) dt (f_n - f_{n-1})
+__device__ __forceinline__ double bdf2_gate(const double *__restrict__ fn, const double *__restrict__ fnm1,
+                                            double dt, int64_t i) {
+  return ((2. / 3.) * dt) * (fn[i] - fnm1[i]);
+}
+
+// guess = base + out * gate
+__global__ void k_guess(const double *__restrict__ un, const double *__restrict__ unm1,
+                        const double *__restrict__ fn, const double *__restrict__ fnm1, double dt,
+                        const float *__restrict__ out, int64_t n, double *__restrict__ guess) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  guess[i] = bdf2_base(un, unm1, fn, fnm1, dt, i) + double(out[i]) * bdf2_gate(fn, fnm1, dt, i);
+}
+
+// loss = mean(d^2), d = (out * gate - (y_{n+1} - base)) / s_o; dZ of the
+// output layer = d loss / d out (bf16, 32 channels, zero beyond NOUT);
+// per-block loss sums
 __global__ void k_loss(const float *__restrict__ out, const double *__restrict__ unp1,
-                       const double *__restrict__ un, const double *__restrict__ unm1, double inv_so,
-                       int64_t npix, __bf16 *__restrict__ dZ, float *__restrict__ lpart) {
+                       const double *__restrict__ un, const double *__restrict__ unm1,
+                       const double *__restrict__ fn, const double *__restrict__ fnm1, double dt,
+                       double inv_so, int64_t npix, __bf16 *__restrict__ dZ, float *__restrict__ lpart) {
   __shared__ float sh[256];
   const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   float l = 0.f;
@@ -255,10 +278,12 @@ __global__ void k_loss(const float *__restrict__ out, const double *__restrict__
     const float scale = 2.f / float(npix * NOUT);
 #pragma unroll
     for (int c = 0; c < NOUT; ++c) {
-      const float t = float((unp1[2 * p + c] - 2. * un[2 * p + c] + unm1[2 * p + c]) * inv_so);
-      const float d = out[p * NOUT + c] - t;
+      const int64_t i = 2 * p + c;
+      const double gs = bdf2_gate(fn, fnm1, dt, i) * inv_so;
+      const double t = (unp1[i] - bdf2_base(un, unm1, fn, fnm1, dt, i)) * inv_so;
+      const float d = float(double(out[p * NOUT + c]) * gs - t);
       l += d * d;
-      dZ[p * C + c] = __bf16(scale * d);
+      dZ[p * C + c] = __bf16(scale * d * float(gs));
     }
     for (int c = NOUT; c < C; ++c) dZ[p * C + c] = __bf16(0.f);
   }
@@ -322,8 +347,9 @@ __global__ void k_pack(const float *__restrict__ w, const int64_t *__restrict__ 
 struct pda_guessnet_s {
   int H = 0, W = 0;
   int64_t npix = 0;
-  double s_d = 1e-2, s_o = 1e-4, nu_mean = 5.5e-3, nu_scale = 4.5e-3;
-  double dt_over_sf = 0.;           // dt / s_f for the velocity features
+  double dt = 5e-3;                 // time step of the base
+  double s_f = 1e-2, s_g = 1e-3;    // spreads of dt f_n and dt (f_n - f_{n-1})
+  double s_o = 1e-6, nu_mean = 5.5e-3, nu_scale = 4.5e-3;
   int64_t nparam = 0;
   int64_t woff[pda::nn::NL + 1] = {0};
   int64_t foff[pda::nn::NL + 1] = {0}, boff[pda::nn::NL + 1] = {0};
@@ -349,10 +375,14 @@ const float *bias_of(const pda_guessnet_s &net, int l) {
   return net.w.as<float>() + net.woff[l] + 9 * cin_of(l) * cout_of(l);
 }
 
-int features(pda_guessnet_s &net, const double *un, const double *unm1, const double *fn, double nu,
-             hipStream_t st) {
-  hipLaunchKernelGGL(k_features, dim3(unsigned((net.npix + 255) / 256)), dim3(256), 0, st, un, unm1, fn,
-                     net.dt_over_sf, (nu - net.nu_mean) / net.nu_scale, 1. / net.s_d, net.npix,
+// the time levels one prediction reads
+struct Levels {
+  const double *un, *unm1, *fn, *fnm1;
+};
+
+int features(pda_guessnet_s &net, const Levels &L, double nu, hipStream_t st) {
+  hipLaunchKernelGGL(k_features, dim3(unsigned((net.npix + 255) / 256)), dim3(256), 0, st, L.un, L.fn, L.fnm1,
+                     net.dt / net.s_f, net.dt / net.s_g, (nu - net.nu_mean) / net.nu_scale, net.npix,
                      net.X0.as<__bf16>());
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
@@ -410,11 +440,10 @@ int backward(pda_guessnet_s &net, hipStream_t st) {
 }
 
 // loss of the current forward against y_{n+1}; leaves the output dZ in dZa
-int loss(pda_guessnet_s &net, const double *un, const double *unm1, const double *unp1, float *lval,
-         hipStream_t st) {
+int loss(pda_guessnet_s &net, const Levels &L, const double *unp1, float *lval, hipStream_t st) {
   const unsigned nb = unsigned((net.npix + 255) / 256);
-  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, net.out.as<float>(), unp1, un, unm1, 1. / net.s_o,
-                     net.npix, net.dZa.as<__bf16>(), net.lpart.as<float>());
+  hipLaunchKernelGGL(k_loss, dim3(nb), dim3(256), 0, st, net.out.as<float>(), unp1, L.un, L.unm1, L.fn, L.fnm1,
+                     net.dt, 1. / net.s_o, net.npix, net.dZa.as<__bf16>(), net.lpart.as<float>());
   PDA_HIP_CHECK(hipGetLastError());
   if (lval) {
     std::vector<float> h(nb);
@@ -513,11 +542,15 @@ int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n) {
   return PDA_OK;
 }
 
-int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_mean, double nu_scale,
-                            double dt_over_sf) {
-  if (!net || !(s_d > 0.) || !(s_o > 0.) || !(nu_scale > 0.) || !(dt_over_sf >= 0.)) return PDA_E_INVALID_ARG;
-  net->dt_over_sf = dt_over_sf;
-  net->s_d = s_d;
+int pda_guessnet_set_scales(pda_guessnet net, double dt, double s_f, double s_g, double s_o, double nu_mean,
+                            double nu_scale) {
+  if (!net || !(dt > 0.) || !(s_f > 0.) || !(s_g > 0.) || !(s_o > 0.) || !(nu_scale > 0.)) {
+    set_error("pda_guessnet_set_scales: dt and the spreads must be positive");
+    return PDA_E_INVALID_ARG;
+  }
+  net->dt = dt;
+  net->s_f = s_f;
+  net->s_g = s_g;
   net->s_o = s_o;
   net->nu_mean = nu_mean;
   net->nu_scale = nu_scale;
@@ -525,27 +558,35 @@ int pda_guessnet_set_scales(pda_guessnet net, double s_d, double s_o, double nu_
 }
 
 int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_fn,
-                         double nu, double *d_guess, pda_stream stream) {
-  if (!net || !d_un || !d_unm1 || !d_guess) return PDA_E_INVALID_ARG;
+                         const double *d_fnm1, double nu, double *d_guess, pda_stream stream) {
+  if (!net || !d_un || !d_unm1 || !d_fn || !d_fnm1 || !d_guess) {
+    set_error("pda_guessnet_predict: null argument");
+    return PDA_E_INVALID_ARG;
+  }
   hipStream_t st = hipStream_t(stream);
-  int rc = features(*net, d_un, d_unm1, d_fn, nu, st);
+  const Levels L{d_un, d_unm1, d_fn, d_fnm1};
+  int rc = features(*net, L, nu, st);
   if (!rc) rc = forward(*net, st);
   if (rc) return rc;
   const int64_t n = net->npix * NOUT;
-  hipLaunchKernelGGL(k_guess, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, d_un, d_unm1,
-                     net->out.as<float>(), net->s_o, n, d_guess);
+  hipLaunchKernelGGL(k_guess, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, d_un, d_unm1, d_fn, d_fnm1,
+                     net->dt, net->out.as<float>(), n, d_guess);
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
 
 int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_fn,
-                           const double *d_unp1, double nu, float *grad, float *lossv, float *outv,
-                           pda_stream stream) {
-  if (!net || !d_un || !d_unm1 || !d_unp1) return PDA_E_INVALID_ARG;
+                           const double *d_fnm1, const double *d_unp1, double nu, float *grad, float *lossv,
+                           float *outv, pda_stream stream) {
+  if (!net || !d_un || !d_unm1 || !d_fn || !d_fnm1 || !d_unp1) {
+    set_error("pda_guessnet_gradients: null argument");
+    return PDA_E_INVALID_ARG;
+  }
   hipStream_t st = hipStream_t(stream);
-  int rc = features(*net, d_un, d_unm1, d_fn, nu, st);
+  const Levels L{d_un, d_unm1, d_fn, d_fnm1};
+  int rc = features(*net, L, nu, st);
   if (!rc) rc = forward(*net, st);
-  if (!rc) rc = loss(*net, d_un, d_unm1, d_unp1, lossv, st);
+  if (!rc) rc = loss(*net, L, d_unp1, lossv, st);
   if (!rc) rc = backward(*net, st);
   if (rc) return rc;
   if (grad)
@@ -557,12 +598,17 @@ int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d
 }
 
 int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1, const double *d_fn,
-                            const double *d_unp1, double nu, float lr, float *lossv, pda_stream stream) {
-  if (!net || !d_un || !d_unm1 || !d_unp1 || !(lr > 0.f)) return PDA_E_INVALID_ARG;
+                            const double *d_fnm1, const double *d_unp1, double nu, float lr, float *lossv,
+                            pda_stream stream) {
+  if (!net || !d_un || !d_unm1 || !d_fn || !d_fnm1 || !d_unp1 || !(lr > 0.f)) {
+    set_error("pda_guessnet_train_step: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
   hipStream_t st = hipStream_t(stream);
-  int rc = features(*net, d_un, d_unm1, d_fn, nu, st);
+  const Levels L{d_un, d_unm1, d_fn, d_fnm1};
+  int rc = features(*net, L, nu, st);
   if (!rc) rc = forward(*net, st);
-  if (!rc) rc = loss(*net, d_un, d_unm1, d_unp1, lossv, st);
+  if (!rc) rc = loss(*net, L, d_unp1, lossv, st);
   if (!rc) rc = backward(*net, st);
   if (rc) return rc;
   const float b1 = 0.9f, b2 = 0.999f;

@@ -64,7 +64,8 @@ __global__ void __launch_bounds__(TB) k_dot2(const double *__restrict__ a, const
 // y = A x (scalar CSR) with LPR lanes per row: the lanes of a row read its
 // values and column indices as contiguous runs; a row's sum is reduced with
 // lane shuffles.  Grid-stride over row groups (at most MAXB blocks), with
-// optional per-block partial dots (d1 . y) and (y . y).
+// optional per-block partial dots (d1 . y) and (y . y).  (Reading one column
+// index per m x m block instead of one per value measured no faster.)
 constexpr int MAXB = 2048;
 
 template <int LPR>
@@ -328,6 +329,17 @@ struct pda_implicit_s {
   pda::DevBuf diag;         // position of the diagonal entry of every scalar row
   int lpr = 32;             // SpMV lanes per row (power of two >= mean row length)
   pda_implicit_stats last{};
+  // check_every BiCGSTAB iterations recorded once as a graph (their kernel
+  // arguments are the same for every solve of this stepper)
+  hipStream_t cap = nullptr;
+  hipGraph_t lin_graph = nullptr;
+  hipGraphExec_t lin_exec = nullptr;
+  const double *lin_x = nullptr;    // the solution vector the graph writes
+  ~pda_implicit_s() {
+    if (lin_exec) (void)hipGraphExecDestroy(lin_exec);
+    if (lin_graph) (void)hipGraphDestroy(lin_graph);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
 };
 
 namespace {
@@ -335,8 +347,11 @@ using namespace pda;
 
 // y = A x with the solver's lanes-per-row; returns the number of blocks
 // (the count of partial sums written when part != null)
-int spmv(const pda_implicit_s &ip, const int64_t *rp, const int32_t *ci, const double *A, const double *x,
-         double *y, const double *d1, int want_yy, double *part, const double *scal, hipStream_t st) {
+int spmv(const pda_implicit_s &ip, const double *A, const double *x, double *y, const double *d1, int want_yy,
+         double *part, const double *scal, hipStream_t st) {
+  const pda_problem_s &p = *ip.prob;
+  const int64_t *rp = p.d_rowptr.as<int64_t>();
+  const int32_t *ci = p.d_colidx.as<int32_t>();
   const int64_t n = ip.n;
   const int64_t rows_per_block = TB / ip.lpr;
   const unsigned g = unsigned(std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, MAXB)));
@@ -350,17 +365,34 @@ int spmv(const pda_implicit_s &ip, const int64_t *rp, const int32_t *ci, const d
   return int(g);
 }
 
+// one BiCGSTAB iteration (preconditioned right: y = M^-1 p, z = M^-1 s);
+// every kernel is a no-op once the convergence flag is set
 template <int M>
-int bicgstab_m(pda_implicit_s &ip, const double *b, double *x, hipStream_t st, double rtol,
-               int *iters, double *rel_res) {
-  pda_problem_s &p = *ip.prob;
+void bicgstab_iteration(pda_implicit_s &ip, double *x, hipStream_t st, bool first) {
   const int64_t n = ip.n, nc = ip.ncell;
-  const int64_t *rp = p.d_rowptr.as<int64_t>();
-  const int32_t *ci = p.d_colidx.as<int32_t>();
   const double *A = ip.J.as<double>();
   double *r = ip.r.as<double>(), *r0 = ip.r0.as<double>(), *pp = ip.p.as<double>();
   double *v = ip.v.as<double>(), *s = ip.s.as<double>(), *t = ip.t.as<double>();
   double *y = ip.y.as<double>(), *z = ip.z.as<double>(), *Mi = ip.Minv.as<double>();
+  double *part = ip.part.as<double>(), *scal = ip.scal.as<double>();
+  const unsigned gv = grid_strided(n), gc = grid_of(nc);
+  const int nb = int(gv);
+  hipLaunchKernelGGL((k_p_prec<M>), dim3(gc), dim3(TB), 0, st, r, pp, v, Mi, y, nc, scal, first ? 1 : 0);
+  int ns = spmv(ip, A, y, v, r0, 0, part, scal, st);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, ns, 1, scal, 0.);
+  hipLaunchKernelGGL((k_s_prec<M>), dim3(gc), dim3(TB), 0, st, r, v, Mi, s, z, nc, scal);
+  ns = spmv(ip, A, z, t, s, 1, part, scal, st);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, ns, 2, scal, 0.);
+  hipLaunchKernelGGL(k_xr, dim3(gv), dim3(TB), 0, st, x, r, y, z, s, t, r0, n, part, scal);
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 0, scal, 0.);
+}
+
+template <int M>
+int bicgstab_m(pda_implicit_s &ip, const double *b, double *x, hipStream_t st, double rtol,
+               int *iters, double *rel_res) {
+  const int64_t n = ip.n, nc = ip.ncell;
+  const double *A = ip.J.as<double>();
+  double *r = ip.r.as<double>(), *r0 = ip.r0.as<double>(), *Mi = ip.Minv.as<double>();
   double *part = ip.part.as<double>(), *scal = ip.scal.as<double>();
   const unsigned gv = grid_strided(n), gc = grid_of(nc);
   const int nb = int(gv);
@@ -378,22 +410,53 @@ int bicgstab_m(pda_implicit_s &ip, const double *b, double *x, hipStream_t st, d
   hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 0, scal, rtol);
   PDA_HIP_CHECK(hipGetLastError());
   const int64_t maxit = ip.lin_max > 0 ? ip.lin_max : 2 * n;
+  const int K = ip.check_every;
   double h[S_COUNT];
-  for (int64_t it = 0; it < maxit; ++it) {
-    hipLaunchKernelGGL((k_p_prec<M>), dim3(gc), dim3(TB), 0, st, r, pp, v, Mi, y, nc, scal, it == 0 ? 1 : 0);
-    int ns = spmv(ip, rp, ci, A, y, v, r0, 0, part, scal, st);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, ns, 1, scal, rtol);
-    hipLaunchKernelGGL((k_s_prec<M>), dim3(gc), dim3(TB), 0, st, r, v, Mi, s, z, nc, scal);
-    ns = spmv(ip, rp, ci, A, z, t, s, 1, part, scal, st);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, ns, 2, scal, rtol);
-    hipLaunchKernelGGL(k_xr, dim3(gv), dim3(TB), 0, st, x, r, y, z, s, t, r0, n, part, scal);
-    hipLaunchKernelGGL(k_finish, dim3(1), dim3(TB), 0, st, part, nb, 0, scal, rtol);
-    PDA_HIP_CHECK(hipGetLastError());
-    if ((it + 1) % ip.check_every == 0 || it + 1 == maxit) {
-      PDA_HIP_CHECK(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
-      PDA_HIP_CHECK(hipStreamSynchronize(st));
-      if (h[S_DONE] != 0.) break;
+  // iteration 0 (p = r) eagerly, then chunks of K iterations replayed from
+  // one graph, polling the convergence flag after each chunk; a tail shorter
+  // than K runs eagerly so that maxit is exact
+  bicgstab_iteration<M>(ip, x, st, true);
+  PDA_HIP_CHECK(hipGetLastError());
+  int64_t it = 1;
+  bool done = false;
+  while (it < maxit && !done) {
+    if (maxit - it >= K) {
+      if (ip.lin_exec && ip.lin_x != x) {
+        (void)hipGraphExecDestroy(ip.lin_exec);
+        (void)hipGraphDestroy(ip.lin_graph);
+        ip.lin_exec = nullptr;
+        ip.lin_graph = nullptr;
+      }
+      if (!ip.lin_exec) {
+        if (!ip.cap) PDA_HIP_CHECK(hipStreamCreateWithFlags(&ip.cap, hipStreamNonBlocking));
+        PDA_HIP_CHECK(hipStreamBeginCapture(ip.cap, hipStreamCaptureModeThreadLocal));
+        for (int k = 0; k < K; ++k) bicgstab_iteration<M>(ip, x, ip.cap, false);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(ip.cap, &g);
+        if (e != hipSuccess || !g) {
+          if (g) (void)hipGraphDestroy(g);
+          set_error(std::string("bicgstab: graph capture: ") + hipGetErrorString(e));
+          return PDA_E_HIP;
+        }
+        const hipError_t ei = hipGraphInstantiate(&ip.lin_exec, g, nullptr, nullptr, 0);
+        if (ei != hipSuccess) {
+          (void)hipGraphDestroy(g);
+          ip.lin_exec = nullptr;
+          set_error(std::string("bicgstab: graph instantiation: ") + hipGetErrorString(ei));
+          return PDA_E_HIP;
+        }
+        ip.lin_graph = g;
+        ip.lin_x = x;
+      }
+      PDA_HIP_CHECK(hipGraphLaunch(ip.lin_exec, st));
+      it += K;
+    } else {
+      for (; it < maxit; ++it) bicgstab_iteration<M>(ip, x, st, false);
+      PDA_HIP_CHECK(hipGetLastError());
     }
+    PDA_HIP_CHECK(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
+    PDA_HIP_CHECK(hipStreamSynchronize(st));
+    done = h[S_DONE] != 0.;
   }
   PDA_HIP_CHECK(hipMemcpyAsync(h, scal, sizeof(h), hipMemcpyDeviceToHost, st));
   PDA_HIP_CHECK(hipStreamSynchronize(st));

@@ -211,4 +211,6 @@ int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_
 // view of the generic kernels: every row (rows == nullptr) or a row list
 View generic_view(const pda_problem_s &p, const pda::DevBuf *rows, int64_t count);
 bool has_box_kernels(const pda_problem_s &p);
+// the velocity march also covers 3D Euler
+bool has_box_velocity(const pda_problem_s &p);
 }  // namespace pda

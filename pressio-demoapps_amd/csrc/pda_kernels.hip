@@ -21,6 +21,92 @@ __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restric
   rows::jacobian_row<Ph, W, WV>(P, int64_t(blockIdx.x) * blockDim.x + threadIdx.x, U, V, J);
 }
 
+// ------------------------------- first-order Euler Jacobian, staged rows
+// Inner rows (no ghost, no BC factor; Euler has no source term) with four
+// lanes per row and 16 rows per wavefront.  Lane a < D evaluates the two
+// faces of axis a.  The Euler flux along axis a is the flux along x with
+// momentum components 1 and 1+a exchanged, so every lane runs the x-face
+// code on exchanged states and writes entry (k, j) of its blocks to
+// (p(k), p(j)), p the 1 <-> 1+a exchange: one instruction stream for all
+// axes.  A row's blocks are assembled in LDS in CSR order (scalar row k,
+// block slot, column), the diagonal block adding the axes in order as the
+// generic row code does, and the wavefront then copies each staged row to
+// its contiguous CSR values with coalesced stores (one thread per row would
+// scatter every store over 64 rows).  Near-boundary rows are left to the
+// generic kernel.
+constexpr int SJ_LANES = 4, SJ_ROWS = 64 / SJ_LANES;
+
+__device__ __forceinline__ int exch(int k, int a) { return k == 1 ? 1 + a : (k == 1 + a ? 1 : k); }
+
+template <int D>
+__global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double *__restrict__ U,
+                                                              double *__restrict__ J) {
+  using Ph = PhEuler<D>;
+  constexpr int M = D + 2, NB = 1 + 2 * D, RS = NB * M, PER = M * RS;
+  __shared__ double stage[SJ_ROWS * PER];
+  __shared__ int64_t row_of[SJ_ROWS];
+  const int lane = threadIdx.x, g = lane / SJ_LANES, a = lane % SJ_LANES;
+  const int64_t t = int64_t(blockIdx.x) * SJ_ROWS + g;
+  int64_t r = -1;
+  if (t < P.nvisit) {
+    const int64_t rr = P.rows ? P.rows[t] : t;
+    if (P.ghost_row[rr] < 0) r = rr;
+  }
+  if (a == 0) row_of[g] = r;
+  const bool act = r >= 0 && a < D;
+  double *st = stage + g * PER;
+  typename Ph::Side Ln, Lp, Rn, Rp;
+  int sL = 0, sR = 0, s0 = 0;
+  double h = 0.;
+  if (act) {
+    const int32_t *gr = P.graph + r * P.ncols;
+    const uint8_t *sl = P.slot + r * P.npos;
+    const int cl = colof<D>(a, 0, 0), cr = colof<D>(a, 1, 0);
+    const int64_t cells[3] = {gr[cl], gr[0], gr[cr]};
+    double q[3][M];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int k = 0; k < M; ++k) q[c][k] = U[cells[c] * M + exch(k, a)];
+    Ph::template sides<0>(P, q[0], q[1], Ln, Lp);
+    Ph::template sides<0>(P, q[1], q[2], Rn, Rp);
+    h = P.hinv[a];
+    sL = sl[cl];
+    sR = sl[cr];
+    s0 = sl[0];
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const int o = exch(k, a) * RS + exch(j, a);
+        st[o + sL * M] = 0. + Ph::template entry<0>(P, Ln, k, j) * h;
+        st[o + sR * M] = 0. + (-Ph::template entry<0>(P, Rp, k, j)) * h;
+      }
+  }
+#pragma unroll
+  for (int ax = 0; ax < D; ++ax) {
+    wave_sync();
+    if (act && a == ax) {
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+          const int o = exch(k, a) * RS + s0 * M + exch(j, a);
+          const double d = (Ph::template entry<0>(P, Lp, k, j) - Ph::template entry<0>(P, Rn, k, j)) * h;
+          st[o] = (ax == 0 ? 0. : st[o]) + d;
+        }
+    }
+  }
+  wave_sync();
+  for (int gg = 0; gg < SJ_ROWS; ++gg) {
+    const int64_t rg = row_of[gg];
+    if (rg < 0) continue;
+    double *dst = J + P.rowbase[rg];
+    const double *src = stage + gg * PER;
+    for (int e = lane; e < PER; e += 64) dst[e] = src[e];
+  }
+}
+
 // ------------------------------------------------- Jacobian action (SpMM)
 // AJ = J * B.  One thread per (scalar row, column of B).
 __global__ void k_apply(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
@@ -120,7 +206,7 @@ static int jac_generic(const View &v, const pda_problem_s &p, const double *U, d
 }
 
 static bool box_velocity(const pda_problem_s &p) {
-  return p.use_box && p.vbox.valid && p.vbox.cells() > 0 && has_box_kernels(p);
+  return p.use_box && p.vbox.valid && p.vbox.cells() > 0 && has_box_velocity(p);
 }
 static bool box_jacobian(const pda_problem_s &p) {
   return p.use_box && p.box.valid && p.box.cells() > 0 && has_box_kernels(p) && !p.view.dup &&
@@ -147,6 +233,17 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
     if (p.view.dup) {
       hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, J, p.nnz, 0.);
       PDA_HIP_CHECK(hipGetLastError());
+    }
+    if (!V && !p.view.dup && p.w == 1 && (p.family == PDA_EULER2D || p.family == PDA_EULER3D)) {
+      // first-order Euler Jacobian alone: staged inner rows, then the
+      // near-boundary rows with the generic row code
+      const View v = generic_view(p, nullptr, 0);
+      const unsigned nb = unsigned((v.nvisit + SJ_ROWS - 1) / SJ_ROWS);
+      if (p.family == PDA_EULER2D) hipLaunchKernelGGL(k_jac_fo_euler_staged<2>, dim3(nb), dim3(64), 0, s, v, U, J);
+      else hipLaunchKernelGGL(k_jac_fo_euler_staged<3>, dim3(nb), dim3(64), 0, s, v, U, J);
+      PDA_HIP_CHECK(hipGetLastError());
+      if (p.n_near == 0) return PDA_OK;
+      return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);
     }
     return jac_generic(generic_view(p, nullptr, 0), p, U, V, J, s);
   }

@@ -341,8 +341,9 @@ int build(pda_problem_s &p) {
   // structured box: inner cells whose scheme stencil does not wrap
   p.box = Box();
   p.h_rem_rows.clear();
-  if (M.structured && dim == 2 &&
-      (p.family == PDA_EULER2D || p.family == PDA_SWE2D || p.family == PDA_BURGERS2D)) {
+  const bool box2d = dim == 2 && (p.family == PDA_EULER2D || p.family == PDA_SWE2D || p.family == PDA_BURGERS2D);
+  const bool box3d = dim == 3 && p.family == PDA_EULER3D;
+  if (M.structured && (box2d || box3d)) {
     const int wm = (M.stencil - 1) / 2;
     bool ok = true;
     for (int a = 0; a < 3; ++a) {
@@ -359,13 +360,15 @@ int build(pda_problem_s &p) {
       p.box.hi[a] = M.n[a] - lo;
       if (p.box.hi[a] <= lo) ok = false;
     }
+    // natural order: row r = (k * ny + j) * nx + i
+    auto outside = [&](const Box &b, int64_t r) {
+      const int64_t i = r % M.n[0], j = (r / M.n[0]) % M.n[1], k = r / (M.n[0] * M.n[1]);
+      return i < b.lo[0] || i >= b.hi[0] || j < b.lo[1] || j >= b.hi[1] || k < b.lo[2] || k >= b.hi[2];
+    };
     if (ok) {
       p.box.valid = true;
-      for (int64_t r = 0; r < nr; ++r) {
-        const int64_t i = r % M.n[0], j = r / M.n[0];
-        if (i < p.box.lo[0] || i >= p.box.hi[0] || j < p.box.lo[1] || j >= p.box.hi[1])
-          p.h_rem_rows.push_back(int32_t(r));
-      }
+      for (int64_t r = 0; r < nr; ++r)
+        if (outside(p.box, r)) p.h_rem_rows.push_back(int32_t(r));
       // velocity box
       p.vbox = p.box;
       for (int a = 0; a < dim; ++a) {
@@ -380,11 +383,8 @@ int build(pda_problem_s &p) {
         }
       }
       p.h_vrem_rows.clear();
-      for (int64_t r = 0; r < nr; ++r) {
-        const int64_t i = r % M.n[0], j = r / M.n[0];
-        if (i < p.vbox.lo[0] || i >= p.vbox.hi[0] || j < p.vbox.lo[1] || j >= p.vbox.hi[1])
-          p.h_vrem_rows.push_back(int32_t(r));
-      }
+      for (int64_t r = 0; r < nr; ++r)
+        if (outside(p.vbox, r)) p.h_vrem_rows.push_back(int32_t(r));
     }
   }
   p.h_ghost_row = std::move(ghost_row);

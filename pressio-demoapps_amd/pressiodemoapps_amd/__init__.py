@@ -94,12 +94,12 @@ def lib():
             "pda_guessnet_num_params": ([vp, P(i64)], i32),
             "pda_guessnet_set_params": ([vp, P(ctypes.c_float), i64], i32),
             "pda_guessnet_get_params": ([vp, P(ctypes.c_float), i64], i32),
-            "pda_guessnet_set_scales": ([vp, dbl, dbl, dbl, dbl, dbl], i32),
-            "pda_guessnet_predict": ([vp, vp, vp, vp, dbl, vp, vp], i32),
-            "pda_guessnet_gradients": ([vp, vp, vp, vp, vp, dbl, P(ctypes.c_float), P(ctypes.c_float),
+            "pda_guessnet_set_scales": ([vp, dbl, dbl, dbl, dbl, dbl, dbl], i32),
+            "pda_guessnet_predict": ([vp, vp, vp, vp, vp, dbl, vp, vp], i32),
+            "pda_guessnet_gradients": ([vp, vp, vp, vp, vp, vp, dbl, P(ctypes.c_float), P(ctypes.c_float),
                                         P(ctypes.c_float), vp], i32),
-            "pda_guessnet_train_step": ([vp, vp, vp, vp, vp, dbl, ctypes.c_float, P(ctypes.c_float), vp],
-                                        i32),
+            "pda_guessnet_train_step": ([vp, vp, vp, vp, vp, vp, dbl, ctypes.c_float, P(ctypes.c_float),
+                                         vp], i32),
             "pda_batch_create": ([P(vp), i32, P(vp)], i32),
             "pda_batch_destroy": ([vp], None),
             "pda_batch_sizes": ([vp, P(i64), P(i64)], i32),
@@ -127,6 +127,11 @@ class InviscidFluxReconstruction(enum.IntEnum):
     FirstOrder = 0
     Weno3 = 1
     Weno5 = 2
+
+
+class ViscousFluxReconstruction(enum.IntEnum):
+    """schemes_info.hpp: the viscous terms are second-order central differences."""
+    FirstOrder = 0
 
 
 class Euler1d(enum.IntEnum):
@@ -474,6 +479,28 @@ def create_problem(mesh, problem_enum, scheme=InviscidFluxReconstruction.FirstOr
     return Problem(mesh, problem_enum, scheme, ic_flag, params)
 
 
+def create_adv_diff_reac_2d_problem_A(mesh, scheme, ux, uy, diffusion, sigma):
+    """Problem A with custom advection velocity, diffusion and reaction
+    (advection_diffusion_reaction2d.hpp:148-171, src_py/main_binder.cc:365-370)."""
+    return Problem(mesh, AdvectionDiffusionReaction2d.ProblemA, scheme, 0,
+                   {"ux": ux, "uy": uy, "diffusion": diffusion, "sigma": sigma})
+
+
+def create_slip_wall_swe_2d_problem(mesh, scheme, gravity, coriolis, pulseMagnitude):
+    """Slip-wall shallow water with custom gravity, Coriolis and pulse
+    (swe2d.hpp:276-302, src_py/main_binder.cc:508-513)."""
+    return Problem(mesh, Swe2d.SlipWall, scheme, 1,
+                   {"gravity": gravity, "coriolis": coriolis, "pulseMagnitude": pulseMagnitude})
+
+
+def create_burgers_2d_problem(mesh, problem_enum, scheme, viscous_scheme, params):
+    """Burgers 2D with a parameter map (advection_diffusion2d.hpp:113-148,
+    src_py/main_binder.cc:394-399)."""
+    if ViscousFluxReconstruction(viscous_scheme) != ViscousFluxReconstruction.FirstOrder:
+        raise PdaError("create_burgers_2d_problem: invalid viscous flux reconstruction")
+    return Problem(mesh, problem_enum, scheme, 1, dict(params))
+
+
 # ------------------------------------------------------- explicit stepping
 class StepScheme(enum.IntEnum):
     ForwardEuler = 0
@@ -592,11 +619,12 @@ class ImplicitStepper:
 
 # ------------------------------------------------------ learned Newton guess
 class GuessNet:
-    """Learned initial guess for implicit steps of a periodic nx x ny
-    two-component field (pda_guessnet_*): guess = 2 y_n - y_{n-1} +
-    s_o * net(y_n, (y_n - y_{n-1})/s_d, dt f(y_n)/s_f, nu_hat), a 4-layer
+    """Learned initial guess for BDF2 steps of a periodic nx x ny
+    two-component field (pda_guessnet_*): guess = base + net(y_n, dt f_n/s_f,
+    dt (f_n - f_{n-1})/s_g, nu_hat) * (2/3) dt (f_n - f_{n-1}) with the FP64
+    base (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1}); a 4-layer
     3x3 conv net with BF16 weights and FP32 accumulation on MFMA, trained on
-    the device.  fn = f(y_n), the problem's velocity at y_n (optional)."""
+    the device.  fn, fnm1 = the problem's velocity at y_n and y_{n-1}."""
 
     def __init__(self, nx, ny):
         h = ctypes.c_void_p()
@@ -606,7 +634,7 @@ class GuessNet:
         n = ctypes.c_int64()
         _check(lib().pda_guessnet_num_params(self._h, ctypes.byref(n)))
         self.nparams = n.value
-        self.scales = (1e-2, 1e-4, 5.5e-3, 4.5e-3, 0.0)
+        self.scales = (5e-3, 1e-2, 1e-3, 1e-6, 5.5e-3, 4.5e-3)
 
     def set_params(self, w):
         w = np.ascontiguousarray(w, dtype=np.float32)
@@ -619,34 +647,30 @@ class GuessNet:
                                              len(w)))
         return w
 
-    def set_scales(self, s_d, s_o, nu_mean, nu_scale, dt_over_sf=0.0):
-        self.scales = (float(s_d), float(s_o), float(nu_mean), float(nu_scale), float(dt_over_sf))
+    def set_scales(self, dt, s_f, s_g, s_o, nu_mean, nu_scale):
+        self.scales = tuple(float(v) for v in (dt, s_f, s_g, s_o, nu_mean, nu_scale))
         _check(lib().pda_guessnet_set_scales(self._h, *self.scales))
 
-    @staticmethod
-    def _opt(t):
-        return _ptr(t) if t is not None else None
-
-    def predict(self, un, unm1, nu, out=None, fn=None):
+    def predict(self, un, unm1, fn, fnm1, nu, out=None):
         if out is None:
             out = _torch().empty_like(un)
-        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), self._opt(fn), float(nu), _ptr(out),
-                                          _stream()))
+        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), float(nu),
+                                          _ptr(out), _stream()))
         return out
 
-    def gradients(self, un, unm1, unp1, nu, fn=None):
+    def gradients(self, un, unm1, fn, fnm1, unp1, nu):
         g = np.zeros(self.nparams, dtype=np.float32)
         o = np.zeros(self.nx * self.ny * 2, dtype=np.float32)
         loss = ctypes.c_float()
         F = ctypes.POINTER(ctypes.c_float)
-        _check(lib().pda_guessnet_gradients(self._h, _ptr(un), _ptr(unm1), self._opt(fn), _ptr(unp1),
+        _check(lib().pda_guessnet_gradients(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), _ptr(unp1),
                                             float(nu), g.ctypes.data_as(F), ctypes.byref(loss),
                                             o.ctypes.data_as(F), _stream()))
         return loss.value, g, o
 
-    def train_step(self, un, unm1, unp1, nu, lr, want_loss=False, fn=None):
+    def train_step(self, un, unm1, fn, fnm1, unp1, nu, lr, want_loss=False):
         loss = ctypes.c_float()
-        _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), self._opt(fn), _ptr(unp1),
+        _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), _ptr(unp1),
                                              float(nu), float(lr),
                                              ctypes.byref(loss) if want_loss else None, _stream()))
         return loss.value if want_loss else None

@@ -18,6 +18,10 @@ def torch():
     return _t
 
 
+DT = 5e-3
+SCALES = (DT, 5e-3, 4e-4, 3e-5, 5.5e-3, 4.5e-3)  # dt, s_f, s_g, s_o, nu_mean, nu_scale
+
+
 def _fields(rng, H, W):
     y, x = np.meshgrid(np.linspace(-1, 1, H, endpoint=False), np.linspace(-1, 1, W, endpoint=False),
                        indexing="ij")
@@ -25,8 +29,10 @@ def _fields(rng, H, W):
                  axis=-1).ravel()
     un = u + 0.01 * rng.standard_normal(u.shape)
     unm1 = un - 0.004 * (1 + rng.standard_normal(u.shape))
-    unp1 = 2 * un - unm1 + 3e-4 * rng.standard_normal(u.shape)
-    return un, unm1, unp1
+    fn = rng.standard_normal(u.shape)
+    fnm1 = fn + 0.08 * rng.standard_normal(u.shape)
+    unp1 = G.base(un, unm1, fn, fnm1, DT) + G.gate(fn, fnm1, DT) * rng.standard_normal(u.shape)
+    return un, unm1, fn, fnm1, unp1
 
 
 def rel(a, b):
@@ -40,14 +46,12 @@ def test_forward_and_gradients_match_oracle(torch):
     net = pda.GuessNet(W, H)
     w = (rng.standard_normal(net.nparams) * 0.15).astype(np.float32)
     net.set_params(w)
-    scales = (1e-2, 1e-3, 5.5e-3, 4.5e-3)
-    net.set_scales(*scales, 0.7)
-    un, unm1, unp1 = _fields(rng, H, W)
-    fn = rng.standard_normal(un.shape)
+    net.set_scales(*SCALES)
+    un, unm1, fn, fnm1, unp1 = _fields(rng, H, W)
     t = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
-    loss, g, out = net.gradients(t(un), t(unm1), t(unp1), 0.004, fn=t(fn))
-    lo, oo, go = G.loss_and_grad(w.astype(np.float64), un, unm1, unp1, 0.004, H, W, *scales, bf=True, fn=fn,
-                                 dt_over_sf=0.7)
+    loss, g, out = net.gradients(t(un), t(unm1), t(fn), t(fnm1), t(unp1), 0.004)
+    lo, oo, go = G.loss_and_grad(w.astype(np.float64), un, unm1, fn, fnm1, unp1, 0.004, H, W, *SCALES,
+                                 bf=True)
     print(f"[gpu-nn] out {rel(out, oo):.2e} loss {abs(loss - lo) / lo:.2e}", flush=True)
     assert rel(out, oo) <= 1e-2
     assert abs(loss - lo) <= 1e-2 * lo
@@ -60,19 +64,19 @@ def test_forward_and_gradients_match_oracle(torch):
         o += nW + co
 
 
-def test_predict_is_extrapolation_plus_scaled_output(torch):
+def test_predict_is_base_plus_scaled_output(torch):
     H, W = 32, 32
     rng = np.random.default_rng(2)
     net = pda.GuessNet(W, H)
     w = (rng.standard_normal(net.nparams) * 0.1).astype(np.float32)
     net.set_params(w)
-    net.set_scales(1e-2, 1e-3, 5.5e-3, 4.5e-3)
-    un, unm1, unp1 = _fields(rng, H, W)
+    net.set_scales(*SCALES)
+    un, unm1, fn, fnm1, unp1 = _fields(rng, H, W)
     t = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
-    gs = net.predict(t(un), t(unm1), 0.004)
-    _, _, out = net.gradients(t(un), t(unm1), t(unp1), 0.004)
+    gs = net.predict(t(un), t(unm1), t(fn), t(fnm1), 0.004)
+    _, _, out = net.gradients(t(un), t(unm1), t(fn), t(fnm1), t(unp1), 0.004)
     torch.cuda.synchronize()
-    expect = 2 * un - unm1 + 1e-3 * out.astype(np.float64)
+    expect = G.base(un, unm1, fn, fnm1, DT) + G.gate(fn, fnm1, DT) * out.astype(np.float64)
     assert np.allclose(gs.cpu().numpy(), expect, rtol=0, atol=1e-12)
 
 
@@ -83,15 +87,15 @@ def test_training_reduces_loss(torch):
     net = pda.GuessNet(W, H)
     w = (rng.standard_normal(net.nparams) * 0.05).astype(np.float32)
     net.set_params(w)
-    net.set_scales(1e-2, 1e-3, 5.5e-3, 4.5e-3)
-    un, unm1, _ = _fields(rng, H, W)
+    net.set_scales(*SCALES)
+    un, unm1, fn, fnm1, _ = _fields(rng, H, W)
     # a target the network can represent: a smooth local function of the fields
-    unp1 = 2 * un - unm1 + 1e-3 * 0.3 * np.tanh(un)
+    unp1 = G.base(un, unm1, fn, fnm1, DT) + G.gate(fn, fnm1, DT) * 0.3 * np.tanh(un)
     t = lambda a: torch.from_numpy(a).cuda()  # noqa: E731
-    a, b, c = t(un), t(unm1), t(unp1)
-    first = net.train_step(a, b, c, 0.004, 3e-3, want_loss=True)
+    a, b, f0, f1, c = t(un), t(unm1), t(fn), t(fnm1), t(unp1)
+    first = net.train_step(a, b, f0, f1, c, 0.004, 3e-3, want_loss=True)
     for _ in range(300):
-        net.train_step(a, b, c, 0.004, 3e-3)
-    last = net.train_step(a, b, c, 0.004, 3e-3, want_loss=True)
+        net.train_step(a, b, f0, f1, c, 0.004, 3e-3)
+    last = net.train_step(a, b, f0, f1, c, 0.004, 3e-3, want_loss=True)
     print(f"[gpu-nn] loss {first:.3e} -> {last:.3e}", flush=True)
     assert last < 0.1 * first

@@ -116,6 +116,7 @@ def test_sample_mesh_matches_oracle(case, oracle, torch):
     assert cases.rel_err(Vg, Vo) <= TOL
     assert cases.rel_err(V2g, V2o) <= TOL
     assert cases.rel_err(Jg, Jo) <= TOL
+    assert cases.rel_err(Jvg, Jo) <= TOL
     assert cases.rel_err(Jvg, Jg) <= 1e-14
 
 
@@ -141,3 +142,4 @@ def test_sample_mesh_5pct_sedov3d(oracle, torch):
               f"J {cases.rel_err(Jg, Jo):.2e}", flush=True)
         assert cases.rel_err(Vg, Vo) <= TOL and cases.rel_err(V2g, V2o) <= TOL
         assert cases.rel_err(Jg, Jo) <= TOL
+        assert cases.rel_err(Jvg, Jo) <= TOL  # jacobian() alone (staged first-order kernel)

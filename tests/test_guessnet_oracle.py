@@ -23,9 +23,10 @@ def test_oracle_gradient_matches_finite_differences():
     un = rng.standard_normal(H * W * 2)
     unm1 = un + 0.01 * rng.standard_normal(H * W * 2)
     unp1 = un + 0.02 * rng.standard_normal(H * W * 2)
-    args = (un, unm1, unp1, 0.004, H, W, 1e-2, 1e-2, 5.5e-3, 4.5e-3)
     fn = rng.standard_normal(H * W * 2)
-    kw = {"fn": fn, "dt_over_sf": 0.5}
+    fnm1 = fn + 0.1 * rng.standard_normal(H * W * 2)
+    args = (un, unm1, fn, fnm1, unp1, 0.004, H, W, 5e-3, 1e-2, 1e-3, 1e-2, 5.5e-3, 4.5e-3)
+    kw = {}
     _, _, g = G.loss_and_grad(w, *args, bf=False, **kw)
     idx = rng.choice(len(w), size=60, replace=False)
     h = 1e-6
@@ -56,9 +57,30 @@ def test_oracle_forward_closed_form():
     w[P[3] + (4 * 2 + 0) * 32 + 0] = 1.0
     rng = np.random.default_rng(1)
     un = rng.standard_normal(H * W * 2)
-    X0 = G.features(un, un, 0.0055, H, W, 1e-2, 5.5e-3, 4.5e-3, rb=False)
+    f = rng.standard_normal(H * W * 2)
+    X0 = G.features(un, f, f, 0.0055, H, W, 5e-3, 1e-2, 1e-3, 5.5e-3, 4.5e-3, rb=False)
     out, _ = G.forward(w, X0, bf=False)
     s = lambda z: z / (1 + np.exp(-z))  # noqa: E731
     expect = s(s(s(un.reshape(H, W, 2)[..., 0])))
     assert np.allclose(out[..., 0], expect, rtol=1e-14, atol=1e-14)
     assert np.allclose(out[..., 1], 0.0)
+
+
+def test_base_is_exact_for_quadratic_trajectories():
+    # y' = a + b t has the quadratic solution y0 + a t + b t^2 / 2, which BDF2
+    # integrates exactly; f is linear in t, so its linear extrapolation is
+    # exact too and the base must equal y(t_{n+1}) up to rounding
+    rng = np.random.default_rng(4)
+    y0, a, b = rng.standard_normal((3, 50))
+    y = lambda t: y0 + a * t + 0.5 * b * t * t  # noqa: E731
+    f = lambda t: a + b * t  # noqa: E731
+    for dt, tn in ((5e-3, 0.3), (0.1, 1.7), (1.0, -2.0)):
+        got = G.base(y(tn), y(tn - dt), f(tn), f(tn - dt), dt)
+        assert np.allclose(got, y(tn + dt), rtol=1e-13, atol=1e-13)
+    # a cubic term is not reproduced: the base is second order, not exact
+    c = 0.7
+    yc = lambda t: y(t) + c * t ** 3  # noqa: E731
+    fc = lambda t: f(t) + 3 * c * t ** 2  # noqa: E731
+    dt, tn = 0.1, 1.0
+    err = G.base(yc(tn), yc(tn - dt), fc(tn), fc(tn - dt), dt) - yc(tn + dt)
+    assert np.all(np.abs(err) > 1e-6)

@@ -12,6 +12,7 @@ import cases
 import pyoracle
 import pressiodemoapps_amd as pda
 
+S = pda.InviscidFluxReconstruction
 pytestmark = pytest.mark.skipif(not pyoracle.ref_available(), reason="oracle/_ref not built")
 
 FAM_PARAMS = {
@@ -97,3 +98,31 @@ def test_sample_mesh_oracle_matches_reference(case, oracle, tmp_path):
     Vfull = oracle.velocity(oracle.from_mesh(full), pb, Ufull)
     assert np.array_equal(Vo[::1].reshape(-1, m), Vfull.reshape(-1, m)[gids]) or \
         np.abs(Vo.reshape(-1, m) - Vfull.reshape(-1, m)[gids]).max() <= 1e-14 * np.abs(Vfull).max()
+
+
+def test_custom_parameter_creators_match_reference(tmp_path):
+    """create_slip_wall_swe_2d_problem / create_adv_diff_reac_2d_problem_A /
+    create_burgers_2d_problem give the reference's set-up."""
+    mesh = pda.create_structured_mesh((20, 22), (-5, 5, -5, 5), 7)
+    d = str(tmp_path / "m1")
+    mesh.write(d)
+    p = pda.create_slip_wall_swe_2d_problem(mesh, S.Weno3, 9.0, -2.5, 0.3)
+    ref = pyoracle.RefProblem(d, 4, 0, int(S.Weno3), 1, {"gravity": 9.0, "coriolis": -2.5, "pulseMagnitude": 0.3})
+    U0 = np.zeros(p.totalDofStencilMesh())
+    assert pda.lib().pda_initial_condition_host(p.handle, U0.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.abs(U0 - ref.initial_condition()).max() <= 1e-14
+    assert p.nonZeros() == ref.nnz
+    q = pda.create_adv_diff_reac_2d_problem_A(mesh, S.Weno5, 0.3, -0.2, 0.01, 0.5)
+    refq = pyoracle.RefProblem(d, 6, 0, int(S.Weno5), 0,
+                               {"ux": 0.3, "uy": -0.2, "diffusion": 0.01, "sigma": 0.5})
+    assert q.nonZeros() == refq.nnz and q.totalDofSampleMesh() == refq.rows
+    mb = pda.create_structured_mesh((16, 18), (-1, 1, -1, 1), 7, True)
+    db = str(tmp_path / "m2")
+    mb.write(db)
+    params = {"diffusion": 2e-3, "pulseMagnitude": 0.7, "pulseX": 0.1}
+    b = pda.create_burgers_2d_problem(mb, pda.AdvectionDiffusion2d.BurgersPeriodic, S.FirstOrder,
+                                      pda.ViscousFluxReconstruction.FirstOrder, params)
+    refb = pyoracle.RefProblem(db, 5, 0, int(S.FirstOrder), 1, params)
+    Ub = np.zeros(b.totalDofStencilMesh())
+    assert pda.lib().pda_initial_condition_host(b.handle, Ub.ctypes.data_as(ctypes.c_void_p)) == 0
+    assert np.abs(Ub - refb.initial_condition()).max() <= 1e-14
