@@ -155,7 +155,7 @@ __device__ __forceinline__ void vel_march_body(const View &P, const Box &B, int 
 }
 
 template <class Ph, int W>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march(View P, Box B, int rows_per_strip,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Ph::DIM == 3 ? 2 : 4))) k_vel_march(View P, Box B, int rows_per_strip,
                                                      const double *__restrict__ U,
                                                      double *__restrict__ V) {
   vel_march_body<Ph, W>(P, B, rows_per_strip, xcd_contiguous(int64_t(blockIdx.x), int64_t(gridDim.x)), U, V);
