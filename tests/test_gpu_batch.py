@@ -90,3 +90,37 @@ def test_batched_rk4_matches_single_instances_and_oracle(oracle, torch):
         k4 = f(y + dt * k3)
         y = y + (dt / 6) * k1 + (dt / 3) * k2 + (dt / 3) * k3 + (dt / 6) * k4
     assert cases.rel_err(batch.instance(U, 0).cpu().numpy(), y) <= 1e-11
+
+
+@pytest.mark.parametrize("family", ["burgers", "riemann"])
+def test_batched_velocity_other_families(family, oracle, torch):
+    """Burgers (periodic) and 2D Euler Riemann (mirror ghosts) instances of
+    different sizes and parameters in one batched launch, each against the
+    oracle."""
+    print(f"[gpu-batch] {family}", flush=True)
+    rng = np.random.default_rng(4)
+    inst = []
+    for n in (18, 40, 27, 64):
+        if family == "burgers":
+            params = {"diffusion": float(rng.uniform(1e-3, 1e-2)), "pulseMagnitude": float(rng.uniform(0.5, 1.0))}
+            mesh = pda.create_structured_mesh((n, n), (-1, 1, -1, 1), 7, True)
+            prob = pda.create_problem(mesh, pda.AdvectionDiffusion2d.BurgersPeriodic, S.Weno5, 1, params)
+            orc = ([params["diffusion"]], 5, int(pda.AdvectionDiffusion2d.BurgersPeriodic))
+        else:
+            params = {"gamma": float(rng.uniform(1.3, 1.5))}
+            mesh = pda.create_structured_mesh((n, n + 3), (0, 1, 0, 1), 7)
+            prob = pda.create_problem(mesh, pda.Euler2d.Riemann, S.Weno5, 2, params)
+            orc = ([params["gamma"]], 2, int(pda.Euler2d.Riemann))
+        inst.append((mesh, prob, orc))
+    batch = pda.Batch([p for _, p, _ in inst])
+    U = batch.initialCondition()
+    U = U * torch.from_numpy(1 + 0.01 * rng.uniform(-1, 1, batch.total)).cuda()
+    V = batch.createRightHandSide()
+    batch.rightHandSide(U, 0.0, V)
+    torch.cuda.synchronize()
+    import pyoracle
+    for i, (mesh, prob, (pp, fam, enum)) in enumerate(inst):
+        pb = pyoracle.Oracle.problem(fam, enum, int(S.Weno5), pp)
+        Vo = oracle.velocity(oracle.from_mesh(mesh), pb, batch.instance(U, i).cpu().numpy())
+        e = cases.rel_err(batch.instance(V, i).cpu().numpy(), Vo)
+        assert e <= 1e-12, (i, e)
