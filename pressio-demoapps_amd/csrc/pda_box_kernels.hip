@@ -388,11 +388,30 @@ __device__ __forceinline__ void jac_fo_euler2d_seg(const View &P, const Box &B, 
     load_cell<M>(U, c, qa);
     load_cell<M>(U, c + db, qb);
   }
+  // Consume the first row's states here (an empty asm statement that only
+  // reads them, so the wait for these loads is placed before the loop): the
+  // loop head then only has to wait for the states prefetched in the
+  // previous iteration, which were issued before that row's stores, and
+  // the stores stay in flight while the next row is computed.
+#pragma unroll
+  for (int k = 0; k < M; ++k) asm volatile("" : : "v"(qa[k]), "v"(qb[k]));
   for (int64_t j = j0; j < j1; ++j) {
     swap_mom(yl, qa);
     swap_mom(yl, qb);
     Side S0, S1;
     Ph::sides<0>(P, qa, qb, S0, S1);
+    {
+      // the row's states are consumed: prefetch the next row's now (the last
+      // row reloads itself, so the loads and their wait counts are static),
+      // and read the row's first CSR offset, so that both arrive while the
+      // blocks are formed
+      const int64_t jn = j + 1 < j1 ? j + 1 : j;
+      const int64_t c = jn * nx + iseg + ci + da;
+      load_cell<M>(U, c, qa);
+      load_cell<M>(U, c + db, qb);
+    }
+    const int cell0 = __builtin_amdgcn_readfirstlane(int(j * nx + iseg));
+    const int64_t rb0 = P.rowbase[cell0];
     const Side nS0 = shfl_next(S0); // x lanes: left-state side of the next x face
     {
       const Side X1 = pick(yl, S1, S0);
@@ -441,14 +460,21 @@ __device__ __forceinline__ void jac_fo_euler2d_seg(const View &P, const Box &B, 
     bl0 = S0;
     bl1 = S1;
     wave_sync();
-    if (j + 1 < j1) {
-      const int64_t c = (j + 1) * nx + iseg + ci + da;
-      load_cell<M>(U, c, qa);
-      load_cell<M>(U, c + db, qb);
-    }
-    // copy the row's blocks out: 16-byte chunks, 40 per cell
-    double2 *dst = reinterpret_cast<double2 *>(J + P.rowbase[j * nx + iseg]);
-    for (int q = lane; q < ncell * (PER / 2); q += 64) {
+    // copy the row's blocks out: 16-byte chunks, 40 per cell.  A fixed,
+    // unrolled count of stores per lane (lanes past the end store the
+    // segment's last chunk again, with the same value) lets the compiler
+    // count them, so the next row waits only for its prefetched states and
+    // not for these stores to reach memory.
+    double2 *dst = reinterpret_cast<double2 *>(J + rb0);
+    const int total = ncell * (PER / 2);
+    // (an opaque copy of the lane index: the chunk addresses are recomputed
+    // per row instead of being held in registers across the whole march)
+    int lq = lane;
+    asm volatile("" : "+v"(lq));
+#pragma unroll
+    for (int it = 0; it < (ESEG * (PER / 2) + 63) / 64; ++it) {
+      const int q0 = lq + 64 * it;
+      const int q = q0 < total ? q0 : total - 1;
       const int cc = q / (PER / 2), o = q - cc * (PER / 2);
       const double *src = stage + cc * ESTR + 2 * o;
       dst[q] = make_double2(src[0], src[1]);
