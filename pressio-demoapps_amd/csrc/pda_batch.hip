@@ -157,11 +157,22 @@ int pda_batch_velocity(pda_batch b, const double *d_U, double, double *d_V, pda_
 
 int pda_batch_autotune(pda_batch b, const double *d_U, pda_stream stream, int *rows_per_strip, double *us) {
   if (!b || !d_U) return PDA_E_INVALID_ARG;
+  // the tables are re-uploaded below (new device buffers): a graph recorded
+  // with the old ones must not be replayed, whatever the outcome
+  drop_graph(*b);
   hipStream_t s = hipStream_t(stream);
   double *V = b->k[3].as<double>();
-  hipEvent_t e0, e1;
-  PDA_HIP_CHECK(hipEventCreate(&e0));
-  PDA_HIP_CHECK(hipEventCreate(&e1));
+  // events owned for the whole call (every return path releases them)
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  PDA_HIP_CHECK(hipEventCreate(&ev.e[0]));
+  PDA_HIP_CHECK(hipEventCreate(&ev.e[1]));
+  hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   int best = b->rps;
   double best_us = 1e300;
   for (int cand : {1, 2, 4, 8, 16}) {
@@ -184,11 +195,8 @@ int pda_batch_autotune(pda_batch b, const double *d_U, pda_stream stream, int *r
       best = cand;
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   const int rc = build_tables(*b, best);
   if (rc) return rc;
-  drop_graph(*b);
   b->tuned_us = best_us;
   if (rows_per_strip) *rows_per_strip = best;
   if (us) *us = best_us;
