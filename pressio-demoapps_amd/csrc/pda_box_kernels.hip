@@ -285,7 +285,7 @@ __global__ void __launch_bounds__(64) k_jac_fo_box2d(View P, Box B, const double
     wave_sync();
     const int ng = ncell - 32 * g < 0 ? 0 : (ncell - 32 * g > 32 ? 32 : ncell - 32 * g);
     CT *dst = reinterpret_cast<CT *>(J + base0 + int64_t(32 * g) * PER);
-    for (int q = lane; q < ng * NCH; q += 64) dst[q] = stage[q / NCH][q % NCH];
+    for (int q = lane; q < ng * NCH; q += 64) store_stream(dst + q, stage[q / NCH][q % NCH]);
     wave_sync();
   }
 }
@@ -477,7 +477,7 @@ __device__ __forceinline__ void jac_fo_euler2d_seg(const View &P, const Box &B, 
       const int q = q0 < total ? q0 : total - 1;
       const int cc = q / (PER / 2), o = q - cc * (PER / 2);
       const double *src = stage + cc * ESTR + 2 * o;
-      dst[q] = make_double2(src[0], src[1]);
+      store_stream(dst + q, make_double2(src[0], src[1]));
     }
     wave_sync();
   }

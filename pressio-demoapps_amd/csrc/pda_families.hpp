@@ -227,6 +227,15 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Streaming (non-temporal) stores for Jacobian values: each value is written
+// once and not read again by the kernel, so the stores carry the nt hint and
+// do not displace the states and tables the kernel re-reads from L2.
+__device__ __forceinline__ void store_stream(double *p, double v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void store_stream(double2 *p, double2 v) {
+  typedef double d2v __attribute__((ext_vector_type(2)));
+  __builtin_nontemporal_store(d2v{v.x, v.y}, reinterpret_cast<d2v *>(p));
+}
+
 template <int DIM>
 __device__ __forceinline__ int colof(int a, int side, int L) {
   if (DIM == 1) return 1 + 2 * L + side;
@@ -326,7 +335,7 @@ __device__ __forceinline__ void put_block(const View &P, double *__restrict__ J,
     for (int j = 0; j < M; ++j) {
       double *dst = J + base + int64_t(k) * rs + s * M + j;
       if (P.dup) *dst += b[k][j];
-      else *dst = b[k][j];
+      else store_stream(dst, b[k][j]);
     }
 }
 

@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
     if (rg < 0) continue;
     double *dst = J + P.rowbase[rg];
     const double *src = stage + gg * PER;
-    for (int e = lane; e < PER; e += 64) dst[e] = src[e];
+    for (int e = lane; e < PER; e += 64) store_stream(dst + e, src[e]);
   }
 }
 
