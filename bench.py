@@ -97,7 +97,10 @@ def extra_workloads():
             r = learned.compare(net, 512, 400, seed=77, log=log)
             r["training"] = meta
             out["implicit_burgers"] = r
-            log(f"[bench] implicit Burgers: learned/previous-step speedup {r['speedup_learned_vs_previous_step']:.2f}")
+            log(f"[bench] implicit Burgers: speedup over the previous-step guess: learned "
+                f"{r['speedup_learned_vs_previous_step']:.2f}, FP64 predictor (linear f) "
+                f"{r['speedup_bdf2_predictor_vs_previous_step']:.2f}, (quadratic f) "
+                f"{r['speedup_bdf2_quadratic_predictor_vs_previous_step']:.2f}")
         else:
             out["implicit_burgers"] = {"error": "no trained weights at " + learned.DEFAULT_WEIGHTS}
     except Exception as e:  # reported, never fatal

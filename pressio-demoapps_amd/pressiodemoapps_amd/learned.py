@@ -59,11 +59,21 @@ def bdf2_predictor(yn, ynm1, fn, fnm1, out):
     return out
 
 
+def bdf2_quadratic_predictor(yn, ynm1, fn, fnm1, fnm2, out):
+    """FP64 guess with f(y_{n+1}) extrapolated quadratically in time,
+    (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (3 f_n - 3 f_{n-1} + f_{n-2})."""
+    out.copy_(yn).mul_(4.0 / 3.0).add_(ynm1, alpha=-1.0 / 3.0)
+    out.add_(fn, alpha=2.0 * DT).add_(fnm1, alpha=-2.0 * DT).add_(fnm2, alpha=(2.0 / 3.0) * DT)
+    return out
+
+
 def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_step=None, lin_rtol=1e-12):
     """BDF2 trajectory from the IC.  guess: None (previous step, y_n),
-    "extrap" (2 y_n - y_{n-1}), "bdf2" (bdf2_predictor) or a GuessNet.
-    keep: set of step indices k whose (y_{k-1}, y_k, y_{k+1}, f_k, f_{k-1})
-    are returned.  Returns (final state, per-step stats, kept samples)."""
+    "extrap" (2 y_n - y_{n-1}), "bdf2" (bdf2_predictor), "bdf2q"
+    (bdf2_quadratic_predictor; bdf2_predictor on the step where f_{n-2}
+    does not exist yet) or a GuessNet.  keep: set of step indices k whose
+    (y_{k-1}, y_k, y_{k+1}, f_k, f_{k-1}) are returned.  Returns (final
+    state, per-step stats, kept samples)."""
     import torch
     ip = ImplicitStepper(prob, ImplicitScheme.BDF2)
     ip.set_options(StopRule.Residual, tol, 30, lin_rtol, 0)
@@ -72,18 +82,20 @@ def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_ste
     stats, samples = [], []
     g = torch.empty_like(U)
     need_f = keep is not None or (guess is not None and not (isinstance(guess, str) and guess == "extrap"))
-    fbuf = [torch.empty_like(U), torch.empty_like(U)] if need_f else None
+    fbuf = [torch.empty_like(U) for _ in range(3)] if need_f else None
     for k in range(nsteps):
         yn = U.clone()
-        fn = fnm1 = None
+        fn = fnm1 = fnm2 = None
         if need_f:
-            fn, fnm1 = fbuf[k % 2], fbuf[(k + 1) % 2]
+            fn, fnm1, fnm2 = fbuf[k % 3], fbuf[(k + 2) % 3], fbuf[(k + 1) % 3]
             prob.rightHandSide(yn, k * DT, fn)
         gs = None
         if prev is not None and guess is not None:
             if isinstance(guess, str) and guess == "extrap":
                 torch.mul(yn, 2.0, out=g)
                 g.sub_(prev)
+            elif isinstance(guess, str) and guess == "bdf2q" and k >= 2:
+                bdf2_quadratic_predictor(yn, prev, fn, fnm1, fnm2, g)
             elif isinstance(guess, str):
                 bdf2_predictor(yn, prev, fn, fnm1, g)
             else:
@@ -180,8 +192,9 @@ def load(path, n):
 
 def compare(net, n, nsteps, seed, log=print, lin_rtol=1e-12):
     """Same held-out instance stepped with the previous-step guess, the
-    linear extrapolation, the FP64 BDF2 predictor and the network on top of
-    it; every run converges each step to ||R|| <= TOL with the same Newton
+    linear extrapolation, the FP64 BDF2 predictor (f(y_{n+1}) extrapolated
+    linearly, and quadratically) and the network on top of the linear
+    one; every run converges each step to ||R|| <= TOL with the same Newton
     and linear-solver settings.  Returns a dict of steps/s, Newton and
     linear iterations, and the final-state differences to the previous-step
     (cold start) run."""
@@ -193,7 +206,7 @@ def compare(net, n, nsteps, seed, log=print, lin_rtol=1e-12):
     out = {"params": params, "steps": nsteps, "lin_rtol": lin_rtol}
     finals = {}
     methods = (("previous_step", None), ("extrapolation", "extrap"), ("bdf2_predictor", "bdf2"),
-               ("learned", net))
+               ("bdf2_quadratic_predictor", "bdf2q"), ("learned", net))
     for name, g in methods:
         # warm-up (library set-up, first-touch allocations)
         run_trajectory(prob, 3, guess=g, nu=params["diffusion"], lin_rtol=lin_rtol)
@@ -216,6 +229,7 @@ def compare(net, n, nsteps, seed, log=print, lin_rtol=1e-12):
     out["speedup_learned_vs_previous_step"] = sp("learned", "previous_step")
     out["speedup_learned_vs_bdf2_predictor"] = sp("learned", "bdf2_predictor")
     out["speedup_bdf2_predictor_vs_previous_step"] = sp("bdf2_predictor", "previous_step")
+    out["speedup_bdf2_quadratic_predictor_vs_previous_step"] = sp("bdf2_quadratic_predictor", "previous_step")
     return out
 
 
