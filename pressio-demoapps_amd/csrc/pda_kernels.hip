@@ -107,6 +107,192 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
   }
 }
 
+// ------------------------------------ WENO3/WENO5 Jacobian, staged rows
+// Inner rows of the WENO Jacobian (and, with WV, the velocity) with 2*D
+// lanes per row: lane (a, f) evaluates face f (0 left, 1 right) of axis a,
+// its reconstruction gradients for every dof and the face flux Jacobians.
+// For these families the flux along axis a is the flux along x of the state
+// with two dofs exchanged (AxisPerm), exchanged back, so every lane runs the
+// x-face code on permuted states.  Face f reads the 2W cells at offsets
+// o = f - W + i (i < 2W) and contributes to their blocks
+//   block(o)[k][j] += s_f h (Jn[k][j] gm[j][i] + Jp[k][j] gp[j][i]),
+// s_0 = +1, s_1 = -1 (impl/mixin_directional_flux_balance_jacobian.hpp:142-284
+// sums the same terms, left face then right face per axis).  The blocks are
+// assembled in an LDS stage in CSR order without races: the left face
+// stores the negative offsets and the right face the positive ones, then
+// each adds its part of the other side; the diagonal block and the velocity
+// take the lanes' parts one lane at a time, and the row's first lane applies
+// the family's sources.  Rows then leave as contiguous coalesced streaming
+// stores.  Near-boundary rows and rows with merged columns stay with the
+// generic kernel.  The per-lane live set is one face's gradients and
+// Jacobians (2 M (2W) + 2 M^2 doubles), where one thread per row holds all
+// faces of all axes and spills.
+template <class Ph> struct AxisPerm {
+  __device__ static int p(int k, int a) { return exch(k, a); } // Euler, SWE: (rho u) <-> (rho u_a)
+};
+template <> struct AxisPerm<PhBurgers> {
+  __device__ static int p(int k, int a) { return k == 0 ? a : (k == a ? 0 : k); } // u <-> u_a
+};
+
+// graph column of the cell at signed offset o along axis a (o != 0)
+template <int D> __device__ __forceinline__ int col_at(int a, int o) {
+  return o < 0 ? colof<D>(a, 0, -o - 1) : colof<D>(a, 1, o - 1);
+}
+
+template <class Ph, int W, bool WV>
+__global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__restrict__ U,
+                                                          double *__restrict__ V, double *__restrict__ J) {
+  constexpr int D = Ph::DIM, M = Ph::M, LPR = 2 * D, RPW = 64 / LPR;
+  constexpr int NB = 1 + 2 * W * D, RS = NB * M, PER = M * RS, G = 2 * W;
+  __shared__ double stage[RPW * PER];
+  __shared__ double vst[WV ? RPW * M : 1];
+  __shared__ int64_t row_of[RPW];
+  const int lane = threadIdx.x, g = lane / LPR, role = lane % LPR;
+  const int a = role >> 1, f = role & 1;
+  const bool inwave = g < RPW; // 3D: 60 of the 64 lanes
+  const int64_t t = int64_t(blockIdx.x) * RPW + g;
+  int64_t r = -1;
+  if (inwave && t < P.nvisit) {
+    const int64_t rr = P.rows ? P.rows[t] : t;
+    if (P.ghost_row[rr] < 0) r = rr;
+  }
+  if (inwave && role == 0) row_of[g] = r;
+  const bool act = r >= 0;
+  double *st = stage + (inwave ? g : 0) * PER;
+  const int32_t *gr = P.graph + (act ? r : 0) * P.ncols;
+  const uint8_t *sl = P.slot + (act ? r : 0) * P.npos;
+  const double h = a == 0 ? P.hinv[0] : (a == 1 ? P.hinv[1] : P.hinv[2]);
+  const double sh = f ? -h : h;
+  double gm[M][G], gp[M][G], Jn[M * M], Jp[M * M];
+  double fm[M], fp[M];
+  if (act) {
+    {
+      double q[M][G];
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int o = f - W + i;
+        const int64_t c = gr[o == 0 ? 0 : col_at<D>(a, o)];
+#pragma unroll
+        for (int k = 0; k < M; ++k) q[k][i] = U[c * M + AxisPerm<Ph>::p(k, a)];
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        if constexpr (W == 2) phys::weno3_grad(q[k], fm[k], fp[k], gm[k], gp[k]);
+        else phys::weno5_grad(q[k], fm[k], fp[k], gm[k], gp[k]);
+      }
+    }
+    Ph::template fjac<0>(P, fm, fp, Jn, Jp);
+    if constexpr (WV) {
+      double F[M];
+      Ph::template flux<0>(P, fm, fp, F);
+      // the face's velocity part, held in fm until its turn below
+#pragma unroll
+      for (int k = 0; k < M; ++k) fm[k] = sh * F[k];
+    }
+    // phase 1: left face -> offsets -W..-1, right face -> offsets 1..W
+#pragma unroll
+    for (int m = 0; m < W; ++m) {
+      const int o = f ? 1 + m : m - W;
+      const int s = sl[col_at<D>(a, o)];
+      const bool first = o == -1 || o == 1;
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+#pragma clang fp contract(off)
+          const double gmv = f ? gm[j][W + m] : gm[j][m], gpv = f ? gp[j][W + m] : gp[j][m];
+          double v = 0. + sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
+          if (first) v += Sources<Ph>::offdiag(P, false, a, k, j);
+          st[AxisPerm<Ph>::p(k, a) * RS + s * M + AxisPerm<Ph>::p(j, a)] = v;
+        }
+    }
+  }
+  wave_sync();
+  if (act) {
+    // phase 2: left face adds to offsets 1..W-1, right face to -W+1..-1
+#pragma unroll
+    for (int m = 0; m + 1 < W; ++m) {
+      const int o = f ? m + 1 - W : m + 1;
+      const int s = sl[col_at<D>(a, o)];
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+#pragma clang fp contract(off)
+          const double gmv = f ? gm[j][m] : gm[j][W + 1 + m], gpv = f ? gp[j][m] : gp[j][W + 1 + m];
+          const int e = AxisPerm<Ph>::p(k, a) * RS + s * M + AxisPerm<Ph>::p(j, a);
+          st[e] += sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
+        }
+    }
+  }
+  // diagonal block and velocity: the row's lanes one at a time, in axis
+  // order, left face before right face
+  const int s0 = act ? sl[0] : 0;
+#pragma unroll
+  for (int ph = 0; ph < LPR; ++ph) {
+    wave_sync();
+    if (act && role == ph) {
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+#pragma clang fp contract(off)
+          const double gmv = f ? gm[j][W - 1] : gm[j][W], gpv = f ? gp[j][W - 1] : gp[j][W];
+          const double d = sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
+          const int e = AxisPerm<Ph>::p(k, a) * RS + s0 * M + AxisPerm<Ph>::p(j, a);
+          st[e] = (ph == 0 ? 0. : st[e]) + d;
+        }
+      if constexpr (WV) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const int e = g * M + AxisPerm<Ph>::p(k, a);
+          vst[e] = (ph == 0 ? 0. : vst[e]) + fm[k];
+        }
+      }
+    }
+  }
+  wave_sync();
+  if (act && role == 0) {
+    double self[M][M], acc[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      acc[k] = WV ? vst[g * M + k] : 0.;
+#pragma unroll
+      for (int j = 0; j < M; ++j) self[k][j] = st[k * RS + s0 * M + j];
+    }
+    Sources<Ph>::apply(P, r, false, U, gr[0], acc, &self[0][0]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) st[k * RS + s0 * M + j] = self[k][j];
+      if constexpr (WV) V[r * M + k] = acc[k];
+    }
+  }
+  wave_sync();
+  for (int gg = 0; gg < RPW; ++gg) {
+    const int64_t rg = row_of[gg];
+    if (rg < 0) continue;
+    double *dst = J + P.rowbase[rg];
+    const double *src = stage + gg * PER;
+    for (int e = lane; e < PER; e += 64) store_stream(dst + e, src[e]);
+  }
+}
+
+template <class Ph, int W>
+int launch_jac_weno_staged(const View &v, const double *U, double *V, double *J, hipStream_t s) {
+  constexpr int RPW = 64 / (2 * Ph::DIM);
+  const unsigned nb = unsigned((v.nvisit + RPW - 1) / RPW);
+  if (V) hipLaunchKernelGGL((k_jac_weno_staged<Ph, W, true>), dim3(nb), dim3(64), 0, s, v, U, V, J);
+  else hipLaunchKernelGGL((k_jac_weno_staged<Ph, W, false>), dim3(nb), dim3(64), 0, s, v, U, V, J);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+template <class Ph>
+int jac_weno_staged(const View &v, int w, const double *U, double *V, double *J, hipStream_t s) {
+  return w == 2 ? launch_jac_weno_staged<Ph, 2>(v, U, V, J, s) : launch_jac_weno_staged<Ph, 3>(v, U, V, J, s);
+}
+
 // ------------------------------------------------- Jacobian action (SpMM)
 // AJ = J * B.  One thread per (scalar row, column of B).
 __global__ void k_apply(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
@@ -233,6 +419,22 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
     if (p.view.dup) {
       hipLaunchKernelGGL(k_fill, dim3(1024), dim3(256), 0, s, J, p.nnz, 0.);
       PDA_HIP_CHECK(hipGetLastError());
+    }
+    if (!p.view.dup && p.w > 1 && p.family != PDA_ADR2D) {
+      // WENO Jacobian: staged inner rows, then the near-boundary rows (first
+      // order with BC factors) with the generic row code
+      const View v = generic_view(p, nullptr, 0);
+      int rc;
+      switch (p.family) {
+        case PDA_EULER1D: rc = jac_weno_staged<PhEuler1>(v, p.w, U, V, J, s); break;
+        case PDA_EULER2D: rc = jac_weno_staged<PhEuler2>(v, p.w, U, V, J, s); break;
+        case PDA_EULER3D: rc = jac_weno_staged<PhEuler3>(v, p.w, U, V, J, s); break;
+        case PDA_SWE2D: rc = jac_weno_staged<PhSwe>(v, p.w, U, V, J, s); break;
+        case PDA_BURGERS2D: rc = jac_weno_staged<PhBurgers>(v, p.w, U, V, J, s); break;
+        default: set_error("jacobian: unsupported family"); return PDA_E_UNSUPPORTED;
+      }
+      if (rc || p.n_near == 0) return rc;
+      return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);
     }
     if (!V && !p.view.dup && p.w == 1 && (p.family == PDA_EULER2D || p.family == PDA_EULER3D)) {
       // first-order Euler Jacobian alone: staged inner rows, then the
