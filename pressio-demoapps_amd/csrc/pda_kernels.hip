@@ -128,15 +128,59 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
 // Jacobians (2 M (2W) + 2 M^2 doubles), where one thread per row holds all
 // faces of all axes and spills.
 template <class Ph> struct AxisPerm {
-  __device__ static int p(int k, int a) { return exch(k, a); } // Euler, SWE: (rho u) <-> (rho u_a)
+  // Euler, SWE: (rho u) <-> (rho u_a)
+  __host__ __device__ static constexpr int p(int k, int a) { return k == 1 ? 1 + a : (k == 1 + a ? 1 : k); }
 };
 template <> struct AxisPerm<PhBurgers> {
-  __device__ static int p(int k, int a) { return k == 0 ? a : (k == a ? 0 : k); } // u <-> u_a
+  __host__ __device__ static constexpr int p(int k, int a) { return k == 0 ? a : (k == a ? 0 : k); } // u <-> u_a
 };
+
+// value of the axis-a frame array x at the index of dof k of the
+// physical frame: x[p_a(k)], selected among the D compile-time candidates
+// (a is the lane's axis), so every index stays a compile-time constant
+template <class Ph, class X>
+__device__ __forceinline__ double perm_pick(const X &x, int a, int k) {
+  double v = x(AxisPerm<Ph>::p(k, 0));
+  if constexpr (Ph::DIM > 1) v = a == 1 ? x(AxisPerm<Ph>::p(k, 1)) : v;
+  if constexpr (Ph::DIM > 2) v = a == 2 ? x(AxisPerm<Ph>::p(k, 2)) : v;
+  return v;
+}
+
+// store (or add) the axis-frame block b at dst (row stride RS) in the
+// physical frame: dst[K][J] = b[p_a(K)][p_a(J)] (p_a is an involution)
+template <class Ph, int RS, bool ADD>
+__device__ __forceinline__ void put_perm(double *dst, int a, const double (&b)[Ph::M][Ph::M]) {
+  constexpr int M = Ph::M;
+#pragma unroll
+  for (int K = 0; K < M; ++K)
+#pragma unroll
+    for (int J = 0; J < M; ++J) {
+      double v = b[AxisPerm<Ph>::p(K, 0)][AxisPerm<Ph>::p(J, 0)];
+      if constexpr (Ph::DIM > 1) v = a == 1 ? b[AxisPerm<Ph>::p(K, 1)][AxisPerm<Ph>::p(J, 1)] : v;
+      if constexpr (Ph::DIM > 2) v = a == 2 ? b[AxisPerm<Ph>::p(K, 2)][AxisPerm<Ph>::p(J, 2)] : v;
+      if constexpr (ADD) dst[K * RS + J] += v;
+      else dst[K * RS + J] = v;
+    }
+}
 
 // graph column of the cell at signed offset o along axis a (o != 0)
 template <int D> __device__ __forceinline__ int col_at(int a, int o) {
   return o < 0 ? colof<D>(a, 0, -o - 1) : colof<D>(a, 1, o - 1);
+}
+
+// one face's contribution to the block of its stencil point i, in the axis
+// frame: s_f h (Jn diag(gm[:, i]) + Jp diag(gp[:, i]))
+template <int M, int G>
+__device__ __forceinline__ void face_block(double sh, const double (&Jn)[M * M], const double (&Jp)[M * M],
+                                           const double (&gm)[M][G], const double (&gp)[M][G], int i0, int i1,
+                                           bool sel, double (&b)[M][M]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int j = 0; j < M; ++j) {
+    const double gmv = sel ? gm[j][i1] : gm[j][i0], gpv = sel ? gp[j][i1] : gp[j][i0];
+#pragma unroll
+    for (int k = 0; k < M; ++k) b[k][j] = sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
+  }
 }
 
 template <class Ph, int W, bool WV>
@@ -172,8 +216,11 @@ __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__
       for (int i = 0; i < G; ++i) {
         const int o = f - W + i;
         const int64_t c = gr[o == 0 ? 0 : col_at<D>(a, o)];
+        double u[M];
 #pragma unroll
-        for (int k = 0; k < M; ++k) q[k][i] = U[c * M + AxisPerm<Ph>::p(k, a)];
+        for (int k = 0; k < M; ++k) u[k] = U[c * M + k];
+#pragma unroll
+        for (int k = 0; k < M; ++k) q[k][i] = perm_pick<Ph>([&](int x) { return u[x]; }, a, k);
       }
 #pragma unroll
       for (int k = 0; k < M; ++k) {
@@ -193,18 +240,15 @@ __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__
 #pragma unroll
     for (int m = 0; m < W; ++m) {
       const int o = f ? 1 + m : m - W;
-      const int s = sl[col_at<D>(a, o)];
-      const bool first = o == -1 || o == 1;
+      double b[M][M];
+      face_block<M, G>(sh, Jn, Jp, gm, gp, m, W + m, f != 0, b);
+      if (o == -1 || o == 1) {
 #pragma unroll
-      for (int k = 0; k < M; ++k)
+        for (int k = 0; k < M; ++k)
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-#pragma clang fp contract(off)
-          const double gmv = f ? gm[j][W + m] : gm[j][m], gpv = f ? gp[j][W + m] : gp[j][m];
-          double v = 0. + sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
-          if (first) v += Sources<Ph>::offdiag(P, false, a, k, j);
-          st[AxisPerm<Ph>::p(k, a) * RS + s * M + AxisPerm<Ph>::p(j, a)] = v;
-        }
+          for (int j = 0; j < M; ++j) b[k][j] += Sources<Ph>::offdiag(P, false, a, k, j);
+      }
+      put_perm<Ph, RS, false>(st + sl[col_at<D>(a, o)] * M, a, b);
     }
   }
   wave_sync();
@@ -213,16 +257,9 @@ __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__
 #pragma unroll
     for (int m = 0; m + 1 < W; ++m) {
       const int o = f ? m + 1 - W : m + 1;
-      const int s = sl[col_at<D>(a, o)];
-#pragma unroll
-      for (int k = 0; k < M; ++k)
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-#pragma clang fp contract(off)
-          const double gmv = f ? gm[j][m] : gm[j][W + 1 + m], gpv = f ? gp[j][m] : gp[j][W + 1 + m];
-          const int e = AxisPerm<Ph>::p(k, a) * RS + s * M + AxisPerm<Ph>::p(j, a);
-          st[e] += sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
-        }
+      double b[M][M];
+      face_block<M, G>(sh, Jn, Jp, gm, gp, W + 1 + m, m, f != 0, b);
+      put_perm<Ph, RS, true>(st + sl[col_at<D>(a, o)] * M, a, b);
     }
   }
   // diagonal block and velocity: the row's lanes one at a time, in axis
@@ -232,21 +269,15 @@ __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__
   for (int ph = 0; ph < LPR; ++ph) {
     wave_sync();
     if (act && role == ph) {
-#pragma unroll
-      for (int k = 0; k < M; ++k)
-#pragma unroll
-        for (int j = 0; j < M; ++j) {
-#pragma clang fp contract(off)
-          const double gmv = f ? gm[j][W - 1] : gm[j][W], gpv = f ? gp[j][W - 1] : gp[j][W];
-          const double d = sh * (Jn[k * M + j] * gmv + Jp[k * M + j] * gpv);
-          const int e = AxisPerm<Ph>::p(k, a) * RS + s0 * M + AxisPerm<Ph>::p(j, a);
-          st[e] = (ph == 0 ? 0. : st[e]) + d;
-        }
+      double b[M][M];
+      face_block<M, G>(sh, Jn, Jp, gm, gp, W, W - 1, f != 0, b);
+      if (ph == 0) put_perm<Ph, RS, false>(st + s0 * M, a, b);
+      else put_perm<Ph, RS, true>(st + s0 * M, a, b);
       if constexpr (WV) {
 #pragma unroll
         for (int k = 0; k < M; ++k) {
-          const int e = g * M + AxisPerm<Ph>::p(k, a);
-          vst[e] = (ph == 0 ? 0. : vst[e]) + fm[k];
+          const double v = perm_pick<Ph>([&](int x) { return fm[x]; }, a, k);
+          vst[g * M + k] = (ph == 0 ? 0. : vst[g * M + k]) + v;
         }
       }
     }
