@@ -131,6 +131,16 @@ int pda_jacobian_pattern_host(pda_problem p, int64_t *rowptr, int32_t *colidx);
 int pda_velocity(pda_problem p, const double *d_U, double t, double *d_V,
                  pda_stream stream);
 
+/* Velocity tuning (no counterpart in the reference): times the candidate
+   strip heights (rows per workgroup) of the structured velocity kernel on
+   d_U and keeps the fastest for this problem; synchronizes the stream.
+   Without a call the height follows the grid size.  *rows_per_strip = 0
+   when the problem's velocity does not use the structured kernel; *us is
+   the time of one velocity() call with the kept height.  The values of
+   velocity() do not depend on the height. */
+int pda_problem_autotune(pda_problem p, const double *d_U, pda_stream stream,
+                         int *rows_per_strip, double *us);
+
 /* rightHandSideAndJacobian(U, t, V, J) (adapter_cpp.hpp:201-208) when d_V is
    non-NULL, jacobian(U, t, J) (adapter_cpp.hpp:223-229) when it is NULL.
    d_Jvals has nnz entries in the order of pda_jacobian_pattern. */

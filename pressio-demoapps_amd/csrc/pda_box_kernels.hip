@@ -514,11 +514,10 @@ bool has_box_velocity(const pda_problem_s &p) {
 template <class Ph>
 static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
   const Box &B = p.vbox;
-  // strips of rows_per_strip rows: ~8k one-wave groups for the 1024^2 grid
   const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
   const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
-  int64_t rps = (bh * nseg * planes) / 8192;
-  rps = rps < 1 ? 1 : (rps > 64 ? 64 : rps);
+  int64_t rps = velocity_strip_height(B);
+  if (p.vel_rps > 0) rps = p.vel_rps; // measured by pda_problem_autotune
   const dim3 grid{unsigned(nseg * ((bh + rps - 1) / rps) * planes), 1, 1};
   if (p.w == 1) hipLaunchKernelGGL((k_vel_march<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
   else if (p.w == 2) hipLaunchKernelGGL((k_vel_march<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
@@ -577,6 +576,17 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
     case PDA_BURGERS2D: return jac_box<PhBurgers>(p, U, J, V, s, rest_done);
   }
   return PDA_E_UNSUPPORTED;
+}
+
+// default strip height of the single-problem velocity march.  Each strip
+// evaluates one extra row of y faces, so taller strips do less work but
+// give fewer waves; 2D grids aim at ~3 waves per SIMD (1024^2: 6 rows,
+// 2,907 waves on 1,024 SIMDs), 3D grids at ~8k waves.
+int64_t velocity_strip_height(const Box &B) {
+  const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
+  const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
+  int64_t rps = planes > 1 ? (bh * nseg * planes) / 8192 : (bh * nseg + 3071) / 3072;
+  return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
 }
 
 // number of one-wave groups the velocity march uses for a box

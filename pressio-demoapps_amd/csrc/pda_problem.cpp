@@ -647,6 +647,69 @@ int pda_velocity(pda_problem p, const double *d_U, double t, double *d_V, pda_st
   return launch_velocity(*p, d_U, t, d_V, hipStream_t(stream));
 }
 
+int pda_problem_autotune(pda_problem p, const double *d_U, pda_stream stream, int *rows_per_strip,
+                         double *us) {
+  if (!p || !d_U) {
+    set_error("pda_problem_autotune: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  int rc = ensure_device(*p);
+  if (rc) return rc;
+  hipStream_t s = hipStream_t(stream);
+  const bool march = p->use_box && p->vbox.valid && p->vbox.cells() > 0 && has_box_velocity(*p);
+  DevBuf V;
+  PDA_HIP_CHECK(V.alloc(sizeof(double) * size_t(std::max<int64_t>(p->view.nrows * p->m, 1))));
+  struct Events {
+    hipEvent_t e[2] = {nullptr, nullptr};
+    ~Events() {
+      for (hipEvent_t x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  PDA_HIP_CHECK(hipEventCreate(&ev.e[0]));
+  PDA_HIP_CHECK(hipEventCreate(&ev.e[1]));
+  auto time_it = [&](double *t_us) -> int {
+    int r = launch_velocity(*p, d_U, 0., V.as<double>(), s); // warm-up
+    if (r) return r;
+    PDA_HIP_CHECK(hipEventRecord(ev.e[0], s));
+    for (int k = 0; k < 10; ++k) {
+      r = launch_velocity(*p, d_U, 0., V.as<double>(), s);
+      if (r) return r;
+    }
+    PDA_HIP_CHECK(hipEventRecord(ev.e[1], s));
+    PDA_HIP_CHECK(hipEventSynchronize(ev.e[1]));
+    float ms = 0.f;
+    PDA_HIP_CHECK(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+    *t_us = 1e3 * double(ms) / 10.;
+    return PDA_OK;
+  };
+  int best = 0;
+  double best_us = 1e300;
+  if (!march) {
+    p->vel_rps = 0;
+    rc = time_it(&best_us);
+    if (rc) return rc;
+  } else {
+    for (int cand : {1, 2, 3, 4, 5, 6, 8, 12, 16}) {
+      p->vel_rps = cand;
+      double t = 0.;
+      rc = time_it(&t);
+      if (rc) {
+        p->vel_rps = 0;
+        return rc;
+      }
+      if (t < best_us) {
+        best_us = t;
+        best = cand;
+      }
+    }
+    p->vel_rps = best;
+  }
+  if (rows_per_strip) *rows_per_strip = best;
+  if (us) *us = best_us;
+  return PDA_OK;
+}
+
 int pda_jacobian(pda_problem p, const double *d_U, double t, double *d_J, double *d_V,
                  pda_stream stream) {
   if (!p || !d_U || !d_J) {

@@ -79,6 +79,7 @@ def lib():
             "pda_jacobian_pattern": ([vp, P(i64), P(i64), P(vp), P(vp)], i32),
             "pda_jacobian_pattern_host": ([vp, P(i64), P(ctypes.c_int32)], i32),
             "pda_velocity": ([vp, vp, dbl, vp, vp], i32),
+            "pda_problem_autotune": ([vp, vp, vp, P(i32), P(dbl)], i32),
             "pda_jacobian": ([vp, vp, dbl, vp, vp, vp], i32),
             "pda_apply_jacobian": ([vp, vp, dbl, vp, i64, i32, vp, vp], i32),
             "pda_jacobian_blocks": ([vp, P(i32), P(vp), P(vp)], i32),
@@ -425,6 +426,17 @@ class Problem:
         _check(lib().pda_velocity(self._h, _ptr(U), float(t), _ptr(V), _stream()))
 
     velocity = rightHandSide
+
+    def autotune(self, U):
+        """Time the velocity kernel's strip heights on state U and keep the
+        fastest for this problem (pda_problem_autotune); returns (rows per
+        strip, us per velocity call).  The velocity values do not depend on
+        the choice."""
+        _dev_tensor(U, self._nst, "state")
+        r, us = ctypes.c_int(), ctypes.c_double()
+        _check(lib().pda_problem_autotune(self._h, _ptr(U), _stream(), ctypes.byref(r), ctypes.byref(us)))
+        return r.value, us.value
+
     rhs = rightHandSide
     __call__ = rightHandSide
 
