@@ -236,7 +236,13 @@ int pda_implicit_set_options(pda_implicit ip, int stop_rule, double tol, int max
    d_Uprev = y_{n-1} for BDF2 (NULL: this step is BDF1, as pressio's first
    BDF2 step).  d_guess: Newton's initial iterate (NULL: y_n, pressio's
    choice).  Synchronizes the stream (the Newton loop reads norms).
-   Returns PDA_E_STATE when Newton does not converge. */
+   Failures return PDA_E_STATE: a non-finite residual or Newton correction
+   leaves d_U holding y_n (pressio's NonlinearSolveFailure, after which its
+   stepper restores y_n: ode_implicit_stepper_standard.hpp:213-225); reaching
+   max_newton without meeting tol leaves the last iterate in d_U with
+   stats->converged = 0 (pressio stops there without an error,
+   solvers_nonlinear/impl/root_finder.cpp:57-104; here the return code
+   reports it). */
 int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev,
                       const double *d_guess, double t, double dt, pda_stream stream,
                       pda_implicit_stats *stats);

@@ -131,9 +131,10 @@ __global__ void __launch_bounds__(TB) k_finish(const double *__restrict__ part, 
   }
   if (scal[S_DONE] != 0.) return;
   if (stage == 0) {
-    // s1 = r0.r, s2 = r.r
+    // s1 = r0.r, s2 = r.r; a non-finite reduction (a breakdown, or NaN in
+    // the system) also ends the solve, whose result the Newton loop rejects
     scal[S_RR] = s2;
-    if (s2 <= scal[S_TOL2]) {
+    if (s2 <= scal[S_TOL2] || !isfinite(s1) || !isfinite(s2)) {
       scal[S_DONE] = 1.;
       return;
     }
@@ -629,6 +630,16 @@ int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev, const
       PDA_HIP_CHECK(hipGetLastError());
       S.residual_norm = norm2(*ip, R, st);
       if (it == 0) S.initial_residual_norm = S.residual_norm;
+      if (!std::isfinite(S.residual_norm)) {
+        // pressio: ResidualHasNans -> NonlinearSolveFailure, the stepper
+        // restores y_n (solvers_nonlinear/impl/root_finder.cpp:60-70,
+        // ode/impl/ode_implicit_stepper_standard.hpp:213-225); d_U has not
+        // been written yet
+        ip->last = S;
+        if (stats) *stats = S;
+        set_error("pda_implicit_step: residual is not finite");
+        return PDA_E_STATE;
+      }
       if (S.residual_norm <= ip->tol) {
         S.converged = 1;
         break;
@@ -644,6 +655,12 @@ int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev, const
       PDA_HIP_CHECK(hipGetLastError());
       S.residual_norm = norm2(*ip, R, st);
       if (it == 0) S.initial_residual_norm = S.residual_norm;
+      if (!std::isfinite(S.residual_norm)) {
+        ip->last = S;
+        if (stats) *stats = S;
+        set_error("pda_implicit_step: residual is not finite");
+        return PDA_E_STATE;
+      }
     }
     // A = cnp1 I + cf df/dy; solve A dx = -R
     hipLaunchKernelGGL(k_scale, dim3(grid_strided(p.nnz)), dim3(TB), 0, st, J, p.nnz, cf);
@@ -666,6 +683,8 @@ int pda_implicit_step(pda_implicit ip, double *d_U, const double *d_Uprev, const
     S.newton_iters = it + 1;
     S.correction_norm = norm2(*ip, dx, st);
     if (!std::isfinite(S.correction_norm)) {
+      ip->last = S;
+      if (stats) *stats = S;
       set_error("pda_implicit_step: Newton correction is not finite");
       return PDA_E_STATE;
     }

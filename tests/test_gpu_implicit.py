@@ -110,3 +110,26 @@ def test_adr_crank_nicolson_matches_reference_gold(scheme, st, name, torch):
     Uh = U.cpu().numpy()
     print(f"[gpu-implicit] adr {name}: max |U-gold| {np.abs(Uh - gold).max():.2e}", flush=True)
     assert np.allclose(Uh, gold, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("stop", ["residual", "correction"])
+def test_nonfinite_state_leaves_yn(stop, torch):
+    """A NaN in y_n makes the residual non-finite: the step fails with
+    PDA_E_STATE and the state still holds y_n (pressio's
+    NonlinearSolveFailure, after which its stepper restores y_n,
+    ode_implicit_stepper_standard.hpp:213-225)."""
+    print(f"[gpu-implicit] non-finite state, {stop} rule", flush=True)
+    mesh = pda.create_structured_mesh((32, 32), (-1.0, 1.0, -1.0, 1.0), 7, True)
+    prob = pda.create_problem(mesh, pda.AdvectionDiffusion2d.BurgersPeriodic,
+                              pda.InviscidFluxReconstruction.Weno5, 1, {"diffusion": 0.005})
+    ip = pda.ImplicitStepper(prob, pda.ImplicitScheme.BDF1)
+    rule = pda.StopRule.Residual if stop == "residual" else pda.StopRule.Correction
+    ip.set_options(rule, 1e-10, 30, 1e-12, 0)
+    U = prob.initialCondition()
+    U[5] = float("nan")
+    U0 = U.clone()
+    with pytest.raises(pda.PdaError, match="not finite"):
+        ip.step(U, 0.0, 5e-3)
+    torch.cuda.synchronize()
+    assert torch.equal(torch.isnan(U), torch.isnan(U0))
+    assert torch.equal(torch.nan_to_num(U), torch.nan_to_num(U0))
