@@ -38,8 +38,15 @@ constexpr int SJ_LANES = 4, SJ_ROWS = 64 / SJ_LANES;
 
 __device__ __forceinline__ int exch(int k, int a) { return k == 1 ? 1 + a : (k == 1 + a ? 1 : k); }
 
-template <int D>
-__global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double *__restrict__ U,
+//
+// With BOX the rows are the cells of a structured box of a natural-order
+// full mesh whose first-order stencil stays in the grid (no ghost, no BC
+// factor): row r = (k ny + j) nx + i, neighbours r -+ stride_a, and the
+// sorted CSR columns put the block of r - stride_a at slot D-1-a, the
+// diagonal at D and r + stride_a at D+1+a, so a row needs no graph, ghost or
+// slot lookups before its state loads.
+template <int D, bool BOX>
+__global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, Box B, const double *__restrict__ U,
                                                               double *__restrict__ J) {
   using Ph = PhEuler<D>;
   constexpr int M = D + 2, NB = 1 + 2 * D, RS = NB * M, PER = M * RS;
@@ -48,7 +55,13 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
   const int lane = threadIdx.x, g = lane / SJ_LANES, a = lane % SJ_LANES;
   const int64_t t = int64_t(blockIdx.x) * SJ_ROWS + g;
   int64_t r = -1;
-  if (t < P.nvisit) {
+  if constexpr (BOX) {
+    const int64_t bw = B.hi[0] - B.lo[0], bh = B.hi[1] - B.lo[1];
+    if (t < bw * bh * (B.hi[2] - B.lo[2])) {
+      const int64_t i = B.lo[0] + t % bw, j = B.lo[1] + (t / bw) % bh, k = B.lo[2] + t / (bw * bh);
+      r = (k * B.n[1] + j) * B.n[0] + i;
+    }
+  } else if (t < P.nvisit) {
     const int64_t rr = P.rows ? P.rows[t] : t;
     if (P.ghost_row[rr] < 0) r = rr;
   }
@@ -59,10 +72,26 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
   int sL = 0, sR = 0, s0 = 0;
   double h = 0.;
   if (act) {
-    const int32_t *gr = P.graph + r * P.ncols;
-    const uint8_t *sl = P.slot + r * P.npos;
-    const int cl = colof<D>(a, 0, 0), cr = colof<D>(a, 1, 0);
-    const int64_t cells[3] = {gr[cl], gr[0], gr[cr]};
+    int64_t cells[3];
+    if constexpr (BOX) {
+      const int64_t stride = a == 0 ? 1 : (a == 1 ? B.n[0] : B.n[0] * B.n[1]);
+      cells[0] = r - stride;
+      cells[1] = r;
+      cells[2] = r + stride;
+      sL = D - 1 - a;
+      sR = D + 1 + a;
+      s0 = D;
+    } else {
+      const int32_t *gr = P.graph + r * P.ncols;
+      const uint8_t *sl = P.slot + r * P.npos;
+      const int cl = colof<D>(a, 0, 0), cr = colof<D>(a, 1, 0);
+      cells[0] = gr[cl];
+      cells[1] = gr[0];
+      cells[2] = gr[cr];
+      sL = sl[cl];
+      sR = sl[cr];
+      s0 = sl[0];
+    }
     double q[3][M];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -71,9 +100,6 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
     Ph::template sides<0>(P, q[0], q[1], Ln, Lp);
     Ph::template sides<0>(P, q[1], q[2], Rn, Rp);
     h = P.hinv[a];
-    sL = sl[cl];
-    sR = sl[cr];
-    s0 = sl[0];
 #pragma unroll
     for (int k = 0; k < M; ++k)
 #pragma unroll
@@ -98,6 +124,29 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
     }
   }
   wave_sync();
+  // rows consecutive in the CSR (a run along x): the stage is their values
+  // in order, copied as one range with 16-byte stores
+  const int64_t r0 = row_of[0], rl = row_of[SJ_ROWS - 1];
+  bool run = r0 >= 0;
+#pragma unroll
+  for (int gg = 1; gg < SJ_ROWS; ++gg) run = run && row_of[gg] == r0 + gg;
+  int64_t rb0 = 0;
+  if (run) {
+    rb0 = P.rowbase[r0];
+    run = P.rowbase[rl] == rb0 + int64_t(SJ_ROWS - 1) * PER;
+  }
+  if (run) {
+    constexpr int TOT = SJ_ROWS * PER;
+    const int head = int(rb0 & 1); // a leading value that is not 16-byte aligned
+    double *dst = J + rb0;
+    if (head && lane == 0) store_stream(dst, stage[0]);
+    const int npairs = (TOT - head) / 2;
+    double2 *d2 = reinterpret_cast<double2 *>(dst + head);
+    for (int q = lane; q < npairs; q += 64)
+      store_stream(d2 + q, make_double2(stage[head + 2 * q], stage[head + 2 * q + 1]));
+    if (((TOT - head) & 1) && lane == 0) store_stream(dst + TOT - 1, stage[TOT - 1]);
+    return;
+  }
   for (int gg = 0; gg < SJ_ROWS; ++gg) {
     const int64_t rg = row_of[gg];
     if (rg < 0) continue;
@@ -471,9 +520,20 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
       // first-order Euler Jacobian alone: staged inner rows, then the
       // near-boundary rows with the generic row code
       const View v = generic_view(p, nullptr, 0);
+      const Box nobox;
+      if (p.family == PDA_EULER3D && p.use_box && p.box.valid && p.box.cells() > 0) {
+        // the box's rows from their coordinates, the rest with the generic
+        // row code
+        const unsigned nb = unsigned((p.box.cells() + SJ_ROWS - 1) / SJ_ROWS);
+        hipLaunchKernelGGL((k_jac_fo_euler_staged<3, true>), dim3(nb), dim3(64), 0, s, v, p.box, U, J);
+        PDA_HIP_CHECK(hipGetLastError());
+        if (p.h_rem_rows.empty()) return PDA_OK;
+        return jac_generic(generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size())), p, U, V, J, s);
+      }
       const unsigned nb = unsigned((v.nvisit + SJ_ROWS - 1) / SJ_ROWS);
-      if (p.family == PDA_EULER2D) hipLaunchKernelGGL(k_jac_fo_euler_staged<2>, dim3(nb), dim3(64), 0, s, v, U, J);
-      else hipLaunchKernelGGL(k_jac_fo_euler_staged<3>, dim3(nb), dim3(64), 0, s, v, U, J);
+      if (p.family == PDA_EULER2D)
+        hipLaunchKernelGGL((k_jac_fo_euler_staged<2, false>), dim3(nb), dim3(64), 0, s, v, nobox, U, J);
+      else hipLaunchKernelGGL((k_jac_fo_euler_staged<3, false>), dim3(nb), dim3(64), 0, s, v, nobox, U, J);
       PDA_HIP_CHECK(hipGetLastError());
       if (p.n_near == 0) return PDA_OK;
       return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);
