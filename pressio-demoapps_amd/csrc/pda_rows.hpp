@@ -101,10 +101,15 @@ __device__ __forceinline__ void velocity_row(const View &P, int64_t t, const dou
   velocity_axis<Ph, W, 0>(P, gr, g, U, acc);
   if (g >= 0) AxisTerms<Ph>::template velocity<0>(P, gr, U, acc);
   if constexpr (Ph::DIM > 1) {
+    // one axis at a time: its gathered stencil and temporaries are dead
+    // before the next axis loads (the 3D WENO5 row otherwise holds all
+    // three axes' stencils and runs at one wave per SIMD)
+    __builtin_amdgcn_sched_barrier(0);
     velocity_axis<Ph, W, 1>(P, gr, g, U, acc);
     if (g >= 0) AxisTerms<Ph>::template velocity<1>(P, gr, U, acc);
   }
   if constexpr (Ph::DIM > 2) {
+    __builtin_amdgcn_sched_barrier(0);
     velocity_axis<Ph, W, 2>(P, gr, g, U, acc);
     if (g >= 0) AxisTerms<Ph>::template velocity<2>(P, gr, U, acc);
   }
