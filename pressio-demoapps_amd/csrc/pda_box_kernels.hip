@@ -581,11 +581,12 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
 // default strip height of the single-problem velocity march.  Each strip
 // evaluates one extra row of y faces, so taller strips do less work but
 // give fewer waves; 2D grids aim at ~3 waves per SIMD (1024^2: 6 rows,
-// 2,907 waves on 1,024 SIMDs), 3D grids at ~8k waves.
+// 2,907 waves on 1,024 SIMDs), 3D grids at ~20 strips per plane (the
+// measured best: 12 rows at 256^3, 5 at 128^3).
 int64_t velocity_strip_height(const Box &B) {
   const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
   const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
-  int64_t rps = planes > 1 ? (bh * nseg * planes) / 8192 : (bh * nseg + 3071) / 3072;
+  int64_t rps = planes > 1 ? bh / 20 : (bh * nseg + 3071) / 3072;
   return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
 }
 

@@ -690,7 +690,7 @@ int pda_problem_autotune(pda_problem p, const double *d_U, pda_stream stream, in
     rc = time_it(&best_us);
     if (rc) return rc;
   } else {
-    for (int cand : {1, 2, 3, 4, 5, 6, 8, 12, 16}) {
+    for (int cand : {1, 2, 3, 4, 5, 6, 8, 12, 16, 24, 32}) {
       p->vel_rps = cand;
       double t = 0.;
       rc = time_it(&t);

@@ -42,6 +42,7 @@ def measure(mesh, label, reps, log):
     U = U0 * (1.0 + 0.01 * (torch.rand(U0.numel(), dtype=torch.float64, device="cuda", generator=g) * 2 - 1))
     V = pa.createRightHandSide()
     J = pb.createJacobian()
+    rps, _ = pa.autotune(U)  # set-up: the velocity kernel's strip height
     rows = pa.totalDofSampleMesh() // 5
     nnz = J.nonZeros()
     tv = _time(lambda: pa.rightHandSide(U, 0.0, V), reps)
@@ -49,6 +50,7 @@ def measure(mesh, label, reps, log):
     bytes_v = 8 * 5 * (pa.totalDofStencilMesh() // 5 + rows)
     bytes_j = 8 * nnz + 8 * 5 * (pa.totalDofStencilMesh() // 5)
     r = {"rows": rows, "stencil_cells": pa.totalDofStencilMesh() // 5, "jacobian_nnz": nnz,
+         "velocity_rows_per_strip": rps,
          "velocity_us": tv * 1e6, "velocity_rows_per_s": rows / tv, "velocity_GBps": bytes_v / tv / 1e9,
          "jacobian_us": tj * 1e6, "jacobian_rows_per_s": rows / tj, "jacobian_GBps": bytes_j / tj / 1e9,
          "finite": bool(torch.isfinite(V).all().item() and torch.isfinite(J.values).all().item())}
