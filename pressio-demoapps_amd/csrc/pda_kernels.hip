@@ -285,6 +285,12 @@ __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__
 #pragma unroll
       for (int k = 0; k < M; ++k) fm[k] = sh * F[k];
     }
+  }
+  // a fence between the face math and the block products: without it the
+  // products are hoisted into the gradient computation and the 2D Euler
+  // WENO5 kernel needs 512 registers and spills (with it: 236, no spills)
+  wave_sync();
+  if (act) {
     // phase 1: left face -> offsets -W..-1, right face -> offsets 1..W
 #pragma unroll
     for (int m = 0; m < W; ++m) {
