@@ -237,10 +237,17 @@ __device__ __forceinline__ void face_block(double sh, const double (&Jn)[M * M],
   }
 }
 
+// rows per wavefront of the staged WENO kernel: 64 / (2 D), fewer for the
+// 2D Euler WENO5 stage (16 rows would take 26.6 KB of LDS, 5 waves per CU;
+// 12 rows fit 8, matching its 2 waves per SIMD)
+template <class Ph, int W> constexpr int weno_staged_rows() {
+  return (Ph::DIM == 2 && Ph::M == 4 && W == 3) ? 12 : 64 / (2 * Ph::DIM);
+}
+
 template <class Ph, int W, bool WV>
 __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__restrict__ U,
                                                           double *__restrict__ V, double *__restrict__ J) {
-  constexpr int D = Ph::DIM, M = Ph::M, LPR = 2 * D, RPW = 64 / LPR;
+  constexpr int D = Ph::DIM, M = Ph::M, LPR = 2 * D, RPW = weno_staged_rows<Ph, W>();
   constexpr int NB = 1 + 2 * W * D, RS = NB * M, PER = M * RS, G = 2 * W;
   __shared__ double stage[RPW * PER];
   __shared__ double vst[WV ? RPW * M : 1];
@@ -371,7 +378,7 @@ __global__ void __launch_bounds__(64) k_jac_weno_staged(View P, const double *__
 
 template <class Ph, int W>
 int launch_jac_weno_staged(const View &v, const double *U, double *V, double *J, hipStream_t s) {
-  constexpr int RPW = 64 / (2 * Ph::DIM);
+  constexpr int RPW = weno_staged_rows<Ph, W>();
   const unsigned nb = unsigned((v.nvisit + RPW - 1) / RPW);
   if (V) hipLaunchKernelGGL((k_jac_weno_staged<Ph, W, true>), dim3(nb), dim3(64), 0, s, v, U, V, J);
   else hipLaunchKernelGGL((k_jac_weno_staged<Ph, W, false>), dim3(nb), dim3(64), 0, s, v, U, V, J);
