@@ -154,6 +154,20 @@ struct BatchInst {
   int rps = 1;
 };
 
+// An explicit stage fused into a velocity evaluation: with v the velocity
+// of entry i, kout[i] = v (when kout is set) and
+//   out[i] = c[0] x[0][i] + ... + c[nx-1] x[nx-1][i] + cv v,
+// accumulated in that order (the order of the separate stage combination).
+// out must not be the vector the velocity reads.
+struct RkStage {
+  double *kout = nullptr;
+  double *out = nullptr;
+  const double *x[4] = {nullptr, nullptr, nullptr, nullptr};
+  double c[4] = {0., 0., 0., 0.};
+  int nx = 0;
+  double cv = 0.;
+};
+
 }  // namespace pda
 
 struct pda_problem_s {
@@ -196,6 +210,11 @@ int ensure_device(pda_problem_s &p);
 // the scalar CSR pattern (d_rowptr, d_colidx) on the device
 int ensure_pattern_device(pda_problem_s &p);
 int launch_velocity(const pda_problem_s &p, const double *U, double t, double *V, hipStream_t s);
+// velocity of every row with an explicit stage fused into its store, for a
+// problem whose velocity is the generic row kernel alone (no structured
+// box); PDA_E_UNSUPPORTED otherwise
+bool velocity_stage_fits(const pda_problem_s &p);
+int launch_velocity_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s);
 int launch_jacobian(const pda_problem_s &p, const double *U, double t, double *J,
                     double *V, hipStream_t s);
 int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64_t ncols,

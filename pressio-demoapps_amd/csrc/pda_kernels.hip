@@ -15,6 +15,26 @@ __global__ void __launch_bounds__(64) k_velocity(View P, const double *__restric
   rows::velocity_row<Ph, W>(P, int64_t(blockIdx.x) * blockDim.x + threadIdx.x, U, V);
 }
 
+// velocity with an explicit stage fused into the store (RkStage)
+template <class Ph, int W>
+__global__ void __launch_bounds__(64) k_velocity_stage(View P, const double *__restrict__ U, RkStage S) {
+  constexpr int M = Ph::M;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= P.nvisit) return;
+  const int64_t r = P.rows ? P.rows[t] : t;
+  double acc[M];
+  rows::velocity_acc<Ph, W>(P, r, U, acc);
+#pragma unroll
+  for (int k = 0; k < M; ++k) {
+    const int64_t e = r * M + k;
+    if (S.kout) S.kout[e] = acc[k];
+    double o = S.c[0] * S.x[0][e];
+    for (int q = 1; q < S.nx; ++q) o += S.c[q] * S.x[q][e];
+    o += S.cv * acc[k];
+    S.out[e] = o;
+  }
+}
+
 template <class Ph, int W, bool WV>
 __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restrict__ U,
                                                    double *__restrict__ V, double *__restrict__ J) {
@@ -425,6 +445,18 @@ static int vel_dispatch(const View &v, int w, const double *U, double *V, hipStr
 }
 
 template <class Ph>
+static int vel_stage_dispatch(const View &v, int w, const double *U, const RkStage &S, hipStream_t s) {
+  const int bs = 64;
+  if (v.nvisit == 0) return PDA_OK;
+  const unsigned nb = unsigned((v.nvisit + bs - 1) / bs);
+  if (w == 1) hipLaunchKernelGGL((k_velocity_stage<Ph, 1>), dim3(nb), dim3(bs), 0, s, v, U, S);
+  else if (w == 2) hipLaunchKernelGGL((k_velocity_stage<Ph, 2>), dim3(nb), dim3(bs), 0, s, v, U, S);
+  else hipLaunchKernelGGL((k_velocity_stage<Ph, 3>), dim3(nb), dim3(bs), 0, s, v, U, S);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+template <class Ph>
 static int jac_dispatch(const View &v, int w, const double *U, double *V, double *J,
                         hipStream_t s) {
   const int bs = 64; // one-wave groups fit next to the structured kernels
@@ -508,6 +540,26 @@ int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, 
                : vel_generic(generic_view(p, &p.d_vrem_rows, int64_t(p.h_vrem_rows.size())), p, U, V, s);
   if (rc) return rc;
   return launch_velocity_box(p, U, V, s);
+}
+
+bool velocity_stage_fits(const pda_problem_s &p) { return p.view.nrows > 0 && !box_velocity(p); }
+
+int launch_velocity_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s) {
+  if (!velocity_stage_fits(p)) return PDA_E_UNSUPPORTED;
+  if (!S.out || S.nx < 1 || S.nx > 4) {
+    set_error("velocity stage: invalid stage");
+    return PDA_E_INVALID_ARG;
+  }
+  const View v = generic_view(p, nullptr, 0);
+  switch (p.family) {
+    case PDA_EULER1D: return vel_stage_dispatch<PhEuler1>(v, p.w, U, S, s);
+    case PDA_EULER2D: return vel_stage_dispatch<PhEuler2>(v, p.w, U, S, s);
+    case PDA_EULER3D: return vel_stage_dispatch<PhEuler3>(v, p.w, U, S, s);
+    case PDA_SWE2D: return vel_stage_dispatch<PhSwe>(v, p.w, U, S, s);
+    case PDA_BURGERS2D: return vel_stage_dispatch<PhBurgers>(v, p.w, U, S, s);
+    case PDA_ADR2D: return vel_stage_dispatch<PhAdr>(v, p.w, U, S, s);
+    default: set_error("velocity: unsupported family"); return PDA_E_UNSUPPORTED;
+  }
 }
 
 int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, double *V,

@@ -85,17 +85,14 @@ __device__ __forceinline__ void velocity_axis(const View &P, const int32_t *gr, 
   for (int k = 0; k < M; ++k) acc[k] += h * (FL[k] - FR[k]);
 }
 
-// velocity of visited row t (row P.rows[t], or t when the list is absent)
+// velocity of graph row r into acc
 template <class Ph, int W>
-__device__ __forceinline__ void velocity_row(const View &P, int64_t t, const double *__restrict__ U,
-                                             double *__restrict__ V) {
+__device__ __forceinline__ void velocity_acc(const View &P, int64_t r, const double *__restrict__ U,
+                                             double (&acc)[Ph::M]) {
   constexpr int M = Ph::M;
-  if (t >= P.nvisit) return;
-  const int64_t r = P.rows ? P.rows[t] : t;
   const int32_t *gr = P.graph + r * P.ncols;
   const int64_t g = P.ghost_row[r];
   if (g >= 0) fill_row_ghosts<Ph::DIM, M>(P, W, g, gr, U);
-  double acc[M];
 #pragma unroll
   for (int k = 0; k < M; ++k) acc[k] = 0.;
   velocity_axis<Ph, W, 0>(P, gr, g, U, acc);
@@ -114,6 +111,17 @@ __device__ __forceinline__ void velocity_row(const View &P, int64_t t, const dou
     if (g >= 0) AxisTerms<Ph>::template velocity<2>(P, gr, U, acc);
   }
   Sources<Ph>::apply(P, r, g >= 0, U, gr[0], acc, nullptr);
+}
+
+// velocity of visited row t (row P.rows[t], or t when the list is absent)
+template <class Ph, int W>
+__device__ __forceinline__ void velocity_row(const View &P, int64_t t, const double *__restrict__ U,
+                                             double *__restrict__ V) {
+  constexpr int M = Ph::M;
+  if (t >= P.nvisit) return;
+  const int64_t r = P.rows ? P.rows[t] : t;
+  double acc[M];
+  velocity_acc<Ph, W>(P, r, U, acc);
 #pragma unroll
   for (int k = 0; k < M; ++k) V[r * M + k] = acc[k];
 }

@@ -74,12 +74,49 @@ namespace {
 using namespace pda;
 
 // one step U <- step(U) at time t; buffers of the stepper
+RkStage stage(double *kout, double *out, std::initializer_list<std::pair<double, const double *>> terms,
+              double cv) {
+  RkStage S;
+  S.kout = kout;
+  S.out = out;
+  for (auto &x : terms) {
+    S.c[S.nx] = x.first;
+    S.x[S.nx] = x.second;
+    ++S.nx;
+  }
+  S.cv = cv;
+  return S;
+}
+
 int step_once(pda_stepper_s &st, double *U, double t, double dt, hipStream_t s) {
   const pda_problem_s &p = *st.prob;
   double *k[4] = {st.k[0].as<double>(), st.k[1].as<double>(), st.k[2].as<double>(), st.k[3].as<double>()};
+  double *aux = st.aux.as<double>();
+  // SSPRK3 and RK4 on a problem evaluated by the generic row kernel alone:
+  // each velocity launch also forms the next stage state (the stage
+  // formulas of rk_step with each combination's terms in the same order,
+  // the velocity term last), half the launches of a step; a stage never
+  // writes the vector its velocity reads (ping-pong through k[1] / k[3])
+  if (velocity_stage_fits(p) && st.scheme == PDA_SSPRK3) {
+    const double third = 1. / 3., twoThird = 2. / 3., quarter = 1. / 4., threeQuarter = 3. / 4.;
+    double *aux2 = k[1];
+    int rc = launch_velocity_stage(p, U, stage(nullptr, aux, {{1., U}}, dt), s);
+    if (!rc) rc = launch_velocity_stage(p, aux, stage(nullptr, aux2, {{quarter, aux}, {threeQuarter, U}}, quarter * dt), s);
+    if (!rc) rc = launch_velocity_stage(p, aux2, stage(nullptr, U, {{third, U}, {twoThird, aux2}}, twoThird * dt), s);
+    return rc;
+  }
+  if (velocity_stage_fits(p) && st.scheme == PDA_RK4) {
+    const double h2 = dt / 2., h6 = dt / 6., h3 = dt / 3.;
+    double *aux2 = k[3];
+    int rc = launch_velocity_stage(p, U, stage(k[0], aux, {{1., U}}, h2), s);
+    if (!rc) rc = launch_velocity_stage(p, aux, stage(k[1], aux2, {{1., U}}, h2), s);
+    if (!rc) rc = launch_velocity_stage(p, aux2, stage(k[2], aux, {{1., U}}, dt), s);
+    if (!rc) rc = launch_velocity_stage(p, aux, stage(nullptr, U, {{1., U}, {h6, k[0]}, {h3, k[1]}, {h3, k[2]}}, h6), s);
+    return rc;
+  }
   return rk_step(
       st.scheme, [&](const double *y, double tt, double *f) { return launch_velocity(p, y, tt, f, s); }, U,
-      st.aux.as<double>(), k, st.len, t, dt, s);
+      aux, k, st.len, t, dt, s);
 }
 
 }  // namespace
