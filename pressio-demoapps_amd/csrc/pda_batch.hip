@@ -175,7 +175,7 @@ int pda_batch_autotune(pda_batch b, const double *d_U, pda_stream stream, int *r
   hipEvent_t e0 = ev.e[0], e1 = ev.e[1];
   int best = b->rps;
   double best_us = 1e300;
-  for (int cand : {1, 2, 4, 8, 16}) {
+  for (int cand : {1, 2, 3, 4, 6, 8, 12, 16, 24, 32}) {
     int rc = build_tables(*b, cand);
     if (rc) return rc;
     rc = batch_velocity(*b, d_U, V, s); // warm-up
