@@ -173,8 +173,8 @@ def main():
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(the path shards by mesh slabs, see DESIGN.md)"}))
+        print(json.dumps({"error": "one GPU is measured here; --gpus must be 1 (several GPUs would "
+                                   "each own a slab of the mesh with its halo rows)"}))
         sys.exit(2)
 
     import numpy as np
