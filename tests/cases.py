@@ -56,6 +56,12 @@ BIG_CASES = [
      True, 1, {"diffusion": 5e-3}),
     ("burgers-big-FirstOrder", pda.AdvectionDiffusion2d.BurgersPeriodic, S.FirstOrder, (130, 66), (-1, 1, -1, 1), 3,
      True, 1, {"diffusion": 5e-3}),
+    # box lines of 38 cells: the 3D first-order Jacobian copies runs of 16
+    # consecutive rows as one range
+    ("euler3d-sedov-big-FirstOrder", pda.Euler3d.SedovSymmetry, S.FirstOrder, (40, 7, 6), (0, 1.2, 0, 1.2, 0, 1.2),
+     7, (), 0, None),
+    ("euler3d-sedov-big-Weno3", pda.Euler3d.SedovSymmetry, S.Weno3, (40, 7, 6), (0, 1.2, 0, 1.2, 0, 1.2),
+     7, (), 0, None),
 ]
 BIG_IDS = [c[0] for c in BIG_CASES]
 

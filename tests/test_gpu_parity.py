@@ -120,6 +120,36 @@ def test_sample_mesh_matches_oracle(case, oracle, torch):
     assert cases.rel_err(Jvg, Jg) <= 1e-14
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_sample_mesh_contiguous_runs(dim, oracle, torch):
+    """A sample mesh of whole runs of cells along x: the staged first-order
+    Jacobian sees 16 consecutive inner rows in one wavefront and copies them
+    as one range; the values still match the oracle."""
+    print(f"[gpu-parity-sample] contiguous runs, {dim}D first order", flush=True)
+    if dim == 2:
+        full = pda.create_structured_mesh((60, 20), (0.0, 1.0, 0.0, 1.0), 7)
+        nx, ny = 60, 20
+        gids = np.array([j * nx + i for j in range(5, 15) for i in range(3, 57)])
+        enum, ic, params, fam = pda.Euler2d.Riemann, 2, None, 2
+    else:
+        full = pda.create_structured_mesh((40, 8, 7), (0.0, 1.2, 0.0, 1.2, 0.0, 1.2), 7)
+        nx, ny = 40, 8
+        gids = np.array([(k * ny + j) * nx + i for k in range(2, 5) for j in range(2, 6) for i in range(2, 38)])
+        enum, ic, params, fam = pda.Euler3d.SedovSymmetry, 0, None, 3
+    sm = pda.create_sample_mesh(full, gids)
+    prob = pda.create_problem(sm, enum, pda.InviscidFluxReconstruction.FirstOrder, ic, params)
+    U = cases.perturbed_state(prob, 6)
+    Vg, V2g, Jg, Jvg, rp, ci = _run_gpu(prob, U, torch)
+    import pyoracle
+    ms = oracle.from_mesh(sm)
+    pb = pyoracle.Oracle.problem(fam, int(enum), int(pda.InviscidFluxReconstruction.FirstOrder), [1.4])
+    V2o, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, U)
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio)
+    print(f"[gpu-parity-sample] contiguous {dim}D: J {cases.rel_err(Jvg, Jo):.2e}", flush=True)
+    assert cases.rel_err(Jg, Jo) <= TOL
+    assert cases.rel_err(Jvg, Jo) <= TOL
+
+
 def test_sample_mesh_5pct_sedov3d(oracle, torch):
     """3D Sedov with symmetry BCs on a 5 % seeded random sample mesh of a
     24^3 grid, first-order and WENO3."""
