@@ -51,9 +51,10 @@ __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restric
 // axes.  A row's blocks are assembled in LDS in CSR order (scalar row k,
 // block slot, column), the diagonal block adding the axes in order as the
 // generic row code does, and the wavefront then copies each staged row to
-// its contiguous CSR values with coalesced stores (one thread per row would
-// scatter every store over 64 rows).  Near-boundary rows are left to the
-// generic kernel.
+// its contiguous CSR values with coalesced streaming stores (one thread per
+// row would scatter every store over 64 rows); 16 rows consecutive in the
+// CSR leave as one range of 16-byte stores.  Near-boundary rows are left to
+// the generic kernel.
 constexpr int SJ_LANES = 4, SJ_ROWS = 64 / SJ_LANES;
 
 __device__ __forceinline__ int exch(int k, int a) { return k == 1 ? 1 + a : (k == 1 + a ? 1 : k); }
