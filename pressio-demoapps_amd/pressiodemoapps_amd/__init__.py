@@ -401,6 +401,7 @@ class Problem:
         return _torch().zeros(self._ns, dtype=_torch().float64, device="cuda")
 
     createVelocity = createRightHandSide
+    createRhs = createRightHandSide  # adapter_cpp.hpp:116
 
     def jacobian_pattern_host(self):
         """(rowptr int64, colidx int32) as numpy, reference CSR order."""
@@ -454,6 +455,14 @@ class Problem:
         _dev_tensor(V, self._ns, "rhs")
         _check(lib().pda_jacobian(self._h, _ptr(U), float(t), _ptr(self._jvals(J)), _ptr(V),
                                   _stream()))
+
+    def rhsAndJacobian(self, U, t, V, J=None):
+        """adapter_cpp.hpp:210-221: with J as rightHandSideAndJacobian,
+        without it as rightHandSide."""
+        if J is None:
+            self.rightHandSide(U, t, V)
+        else:
+            self.rightHandSideAndJacobian(U, t, V, J)
 
     def createApplyJacobianResult(self, B):
         torch = _torch()

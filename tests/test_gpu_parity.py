@@ -173,3 +173,21 @@ def test_sample_mesh_5pct_sedov3d(oracle, torch):
         assert cases.rel_err(Vg, Vo) <= TOL and cases.rel_err(V2g, V2o) <= TOL
         assert cases.rel_err(Jg, Jo) <= TOL
         assert cases.rel_err(Jvg, Jo) <= TOL  # jacobian() alone (staged first-order kernel)
+
+
+def test_rhs_and_jacobian_aliases(torch):
+    """The adapter's other names (adapter_cpp.hpp:116, :210-221): createRhs,
+    rhsAndJacobian with and without a Jacobian."""
+    case = [c for c in cases.BIG_CASES if c[0] == "euler2d-riemann-big-Weno5"][0]
+    _, prob = cases.make(case)
+    U = torch.from_numpy(cases.perturbed_state(prob, 4)).cuda()
+    V1, V2, V3 = prob.createRhs(), prob.createRightHandSide(), prob.createRhs()
+    J1, J2 = prob.createJacobian(), prob.createJacobian()
+    prob.rhsAndJacobian(U, 0.0, V1)
+    prob.rightHandSide(U, 0.0, V2)
+    prob.rhsAndJacobian(U, 0.0, V3, J1)
+    Vj = prob.createRhs()
+    prob.rightHandSideAndJacobian(U, 0.0, Vj, J2)
+    torch.cuda.synchronize()
+    assert torch.equal(V1, V2)
+    assert torch.equal(V3, Vj) and torch.equal(J1.values, J2.values)
