@@ -223,6 +223,16 @@ class CellCenteredUniformMesh:
     def dz(self):
         return self._d[2]
 
+    # 1/d of the axes the mesh has, 0 for the others (impl/mesh_read_info.hpp:86-92)
+    def dxInv(self):
+        return 1.0 / self._d[0] if self._d[0] != 0.0 else 0.0
+
+    def dyInv(self):
+        return 1.0 / self._d[1] if self._d[1] != 0.0 else 0.0
+
+    def dzInv(self):
+        return 1.0 / self._d[2] if self._d[2] != 0.0 else 0.0
+
     def graph(self):
         ptr = lib().pda_mesh_graph(self._h)
         nc = 1 + (self._stencil - 1) * self._dim

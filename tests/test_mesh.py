@@ -59,6 +59,9 @@ def test_structured_generator_matches_reference_golds(path, tmp_path):
     back = pda.load_cellcentered_uniform_mesh(out)
     assert np.array_equal(back.graph(), mesh.graph())
     assert back.dx() == mesh.dx() and back.dy() == mesh.dy()
+    # inverse spacings as the reader computes them, 0 for an absent axis
+    assert back.dxInv() == 1.0 / back.dx() and back.dyInv() == 1.0 / back.dy()
+    assert back.dzInv() == (1.0 / back.dz() if back.dz() != 0.0 else 0.0)
 
 
 def test_sample_mesh_builder_matches_reference(tmp_path):
