@@ -60,8 +60,8 @@ def test_structured_generator_matches_reference_golds(path, tmp_path):
     assert np.array_equal(back.graph(), mesh.graph())
     assert back.dx() == mesh.dx() and back.dy() == mesh.dy()
     # inverse spacings as the reader computes them, 0 for an absent axis
-    assert back.dxInv() == 1.0 / back.dx() and back.dyInv() == 1.0 / back.dy()
-    assert back.dzInv() == (1.0 / back.dz() if back.dz() != 0.0 else 0.0)
+    for d, inv in ((back.dx(), back.dxInv()), (back.dy(), back.dyInv()), (back.dz(), back.dzInv())):
+        assert inv == (1.0 / d if d != 0.0 else 0.0)
 
 
 def test_sample_mesh_builder_matches_reference(tmp_path):
