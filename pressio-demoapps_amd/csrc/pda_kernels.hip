@@ -135,6 +135,17 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, Box B, const
         st[o + sR * M] = 0. + (-Ph::template entry<0>(P, Rp, k, j)) * h;
       }
   }
+  // every lane forms its axis's part of the diagonal block at once; the
+  // axes then add their parts in order, one axis at a time, in LDS
+  double dg[M][M];
+  wave_sync();
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < M; ++k)
+#pragma unroll
+      for (int j = 0; j < M; ++j)
+        dg[k][j] = (Ph::template entry<0>(P, Lp, k, j) - Ph::template entry<0>(P, Rn, k, j)) * h;
+  }
 #pragma unroll
   for (int ax = 0; ax < D; ++ax) {
     wave_sync();
@@ -144,8 +155,7 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, Box B, const
 #pragma unroll
         for (int j = 0; j < M; ++j) {
           const int o = exch(k, a) * RS + s0 * M + exch(j, a);
-          const double d = (Ph::template entry<0>(P, Lp, k, j) - Ph::template entry<0>(P, Rn, k, j)) * h;
-          st[o] = (ax == 0 ? 0. : st[o]) + d;
+          st[o] = (ax == 0 ? 0. : st[o]) + dg[k][j];
         }
     }
   }
