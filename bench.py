@@ -211,6 +211,31 @@ def main():
         probA.rightHandSide(U, 0.0, V)
         probB.jacobian(U, 0.0, J)
 
+    # autotune() times back-to-back calls on one state, which stays in cache;
+    # in these steps each velocity reads a state that the previous step's
+    # 671 MB Jacobian write has evicted, and a lower strip height hides that
+    # latency better.  So the height is also timed in this loop (untimed
+    # set-up: two passes of 3 + 12 steps per candidate, best pass kept).
+    def ctx_us(h):
+        probA.set_velocity_strip_height(h)
+        best = 1e300
+        for _ in range(2):
+            for i in range(3):
+                step(i)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for i in range(12):
+                step(3 + i)
+            torch.cuda.synchronize()
+            best = min(best, (time.perf_counter() - t) / 12 * 1e6)
+        return best
+    cands = sorted({2, 3, 4, 5, 6, 8, rps})
+    ctx = {h: ctx_us(h) for h in cands}
+    rps = min(ctx, key=ctx.get)
+    probA.set_velocity_strip_height(rps)
+    log(f"[bench] strip height in the step loop: " + ", ".join(f"{h}: {ctx[h]:.0f} us" for h in cands)
+        + f" -> {rps}")
+
     log(f"[bench] warmup {args.warmup}")
     for i in range(args.warmup):
         step(i)

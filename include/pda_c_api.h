@@ -141,6 +141,14 @@ int pda_velocity(pda_problem p, const double *d_U, double t, double *d_V,
 int pda_problem_autotune(pda_problem p, const double *d_U, pda_stream stream,
                          int *rows_per_strip, double *us);
 
+/* Sets the strip height of the structured velocity kernel (no counterpart
+   in the reference), for callers that tune it in their own loop: 0 restores
+   the grid-size rule, 1..64 fixes the height; other values are
+   PDA_E_INVALID_ARG.  Problems whose velocity does not use the structured
+   kernel accept and ignore it.  The values of velocity() do not depend on
+   the height. */
+int pda_problem_set_velocity_strip_height(pda_problem p, int rows_per_strip);
+
 /* rightHandSideAndJacobian(U, t, V, J) (adapter_cpp.hpp:201-208) when d_V is
    non-NULL, jacobian(U, t, J) (adapter_cpp.hpp:223-229) when it is NULL.
    d_Jvals has nnz entries in the order of pda_jacobian_pattern. */

@@ -80,6 +80,7 @@ def lib():
             "pda_jacobian_pattern_host": ([vp, P(i64), P(ctypes.c_int32)], i32),
             "pda_velocity": ([vp, vp, dbl, vp, vp], i32),
             "pda_problem_autotune": ([vp, vp, vp, P(i32), P(dbl)], i32),
+            "pda_problem_set_velocity_strip_height": ([vp, i32], i32),
             "pda_jacobian": ([vp, vp, dbl, vp, vp, vp], i32),
             "pda_apply_jacobian": ([vp, vp, dbl, vp, i64, i32, vp, vp], i32),
             "pda_jacobian_blocks": ([vp, P(i32), P(vp), P(vp)], i32),
@@ -447,6 +448,13 @@ class Problem:
         r, us = ctypes.c_int(), ctypes.c_double()
         _check(lib().pda_problem_autotune(self._h, _ptr(U), _stream(), ctypes.byref(r), ctypes.byref(us)))
         return r.value, us.value
+
+    def set_velocity_strip_height(self, rows_per_strip):
+        """Fix the velocity kernel's strip height (rows per workgroup) for
+        this problem, e.g. after timing heights in the caller's own loop
+        (pda_problem_set_velocity_strip_height); 0 restores the grid-size
+        rule.  The velocity values do not depend on the height."""
+        _check(lib().pda_problem_set_velocity_strip_height(self._h, int(rows_per_strip)))
 
     rhs = rightHandSide
     __call__ = rightHandSide

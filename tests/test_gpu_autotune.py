@@ -49,3 +49,28 @@ def test_autotune_without_structured_kernel(torch):
     rps, us = prob.autotune(U)
     torch.cuda.synchronize()
     assert rps == 0 and us > 0
+
+
+def test_set_strip_height(torch):
+    """pda_problem_set_velocity_strip_height: heights 1, 7, 64 and 0 (the
+    grid-size rule) give bit-identical velocities; heights outside 0..64 are
+    rejected with an error and leave the problem usable."""
+    case = [c for c in cases.BIG_CASES if c[0] == "euler2d-riemann-big-Weno5"][0]
+    print("[gpu-autotune] set strip height", flush=True)
+    mesh, prob = cases.make(case)
+    Ud = torch.from_numpy(cases.perturbed_state(prob, 5)).cuda()
+    V0 = prob.createRightHandSide()
+    prob.rightHandSide(Ud, 0.0, V0)
+    for h in (1, 7, 64, 0):
+        prob.set_velocity_strip_height(h)
+        V = prob.createRightHandSide()
+        prob.rightHandSide(Ud, 0.0, V)
+        torch.cuda.synchronize()
+        assert torch.equal(V, V0), h
+    for bad in (-1, 65):
+        with pytest.raises(pda.PdaError):
+            prob.set_velocity_strip_height(bad)
+    V = prob.createRightHandSide()
+    prob.rightHandSide(Ud, 0.0, V)
+    torch.cuda.synchronize()
+    assert torch.equal(V, V0)
