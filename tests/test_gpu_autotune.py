@@ -49,6 +49,14 @@ def test_autotune_without_structured_kernel(torch):
     rps, us = prob.autotune(U)
     torch.cuda.synchronize()
     assert rps == 0 and us > 0
+    # a fixed height is accepted and ignored: the velocity does not change
+    V0 = prob.createRightHandSide()
+    prob.rightHandSide(U, 0.0, V0)
+    prob.set_velocity_strip_height(5)
+    V1 = prob.createRightHandSide()
+    prob.rightHandSide(U, 0.0, V1)
+    torch.cuda.synchronize()
+    assert torch.equal(V0, V1)
 
 
 def test_set_strip_height(torch):
