@@ -202,6 +202,12 @@ int pda_batch_velocity(pda_batch b, const double *d_U, double t, double *d_V, pd
    velocity kernel on d_U and keeps the fastest; synchronizes the stream */
 int pda_batch_autotune(pda_batch b, const double *d_U, pda_stream stream, int *rows_per_strip,
                        double *us);
+/* fixes the batched velocity kernel's strip height (1..64, else
+   PDA_E_INVALID_ARG), for callers that time heights in their own stepping
+   loop (no counterpart in the reference); synchronizes the device (the
+   tables are rebuilt) and drops a recorded graph; values do not depend on
+   the height */
+int pda_batch_set_velocity_strip_height(pda_batch b, int rows_per_strip);
 /* nsteps explicit steps (PDA_FORWARD_EULER / SSPRK3 / RK4 / RK2) of the whole
    batch; steps_per_graph > 0 records that many steps as one hipGraph
    (re-recorded when d_U, dt, the scheme or the count change) and replays it */

@@ -203,6 +203,17 @@ int pda_batch_autotune(pda_batch b, const double *d_U, pda_stream stream, int *r
   return PDA_OK;
 }
 
+int pda_batch_set_velocity_strip_height(pda_batch b, int rows_per_strip) {
+  if (!b || rows_per_strip < 1 || rows_per_strip > 64) {
+    set_error("pda_batch_set_velocity_strip_height: rows_per_strip must be 1..64");
+    return PDA_E_INVALID_ARG;
+  }
+  // work in flight may still read the tables that are about to be replaced
+  PDA_HIP_CHECK(hipDeviceSynchronize());
+  drop_graph(*b);
+  return build_tables(*b, rows_per_strip);
+}
+
 int pda_batch_advance(pda_batch b, int scheme, double *d_U, double t0, double dt, int64_t nsteps,
                       int steps_per_graph, pda_stream stream) {
   if (!b || !d_U || nsteps < 0 || scheme < PDA_FORWARD_EULER || scheme > PDA_RK2) {

@@ -108,6 +108,7 @@ def lib():
             "pda_batch_initial_condition": ([vp, vp, vp], i32),
             "pda_batch_velocity": ([vp, vp, dbl, vp, vp], i32),
             "pda_batch_autotune": ([vp, vp, vp, P(i32), P(dbl)], i32),
+            "pda_batch_set_velocity_strip_height": ([vp, i32], i32),
             "pda_batch_advance": ([vp, i32, vp, dbl, dbl, i64, i32, vp], i32),
         }
         for name, (args, res) in sig.items():
@@ -761,6 +762,12 @@ class Batch:
         r, us = ctypes.c_int(), ctypes.c_double()
         _check(lib().pda_batch_autotune(self._h, _ptr(U), _stream(), ctypes.byref(r), ctypes.byref(us)))
         return r.value, us.value
+
+    def set_velocity_strip_height(self, rows_per_strip):
+        """Fix the batched velocity kernel's strip height (1..64), e.g. after
+        timing heights in the caller's own stepping loop
+        (pda_batch_set_velocity_strip_height); values do not depend on it."""
+        _check(lib().pda_batch_set_velocity_strip_height(self._h, int(rows_per_strip)))
 
     def advance(self, U, scheme, dt, nsteps, t0=0.0, steps_per_graph=10):
         _dev_tensor(U, self.total, "state")

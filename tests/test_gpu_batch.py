@@ -124,3 +124,24 @@ def test_batched_velocity_other_families(family, oracle, torch):
         Vo = oracle.velocity(oracle.from_mesh(mesh), pb, batch.instance(U, i).cpu().numpy())
         e = cases.rel_err(batch.instance(V, i).cpu().numpy(), Vo)
         assert e <= 1e-12, (i, e)
+
+
+def test_batch_set_strip_height(torch):
+    """pda_batch_set_velocity_strip_height: heights 1, 5 and 64 give the same
+    batched velocity bit for bit; 0 and 65 are rejected."""
+    print("[gpu-batch] set strip height", flush=True)
+    inst = _instances(pda.InviscidFluxReconstruction.Weno5, (16, 24, 33, 64), 3)
+    batch = pda.Batch([p for _, p, _ in inst])
+    rng = np.random.default_rng(4)
+    U = batch.initialCondition() * torch.from_numpy(1 + 0.01 * rng.uniform(-1, 1, batch.total)).cuda()
+    V0 = batch.createRightHandSide()
+    batch.rightHandSide(U, 0.0, V0)
+    for h in (1, 5, 64):
+        batch.set_velocity_strip_height(h)
+        V = batch.createRightHandSide()
+        batch.rightHandSide(U, 0.0, V)
+        torch.cuda.synchronize()
+        assert torch.equal(V, V0), h
+    for bad in (0, 65):
+        with pytest.raises(pda.PdaError):
+            batch.set_velocity_strip_height(bad)
