@@ -546,8 +546,8 @@ static int jac_box(const pda_problem_s &p, const double *U, double *J, double *V
       // strips of rows_per_strip rows: ~16k one-wave groups for the 1024^2 grid
       const int64_t nseg = (B.hi[0] - B.lo[0] + ESEG - 1) / ESEG;
       const int64_t bh = B.hi[1] - B.lo[1];
-      int64_t rps = (bh * nseg) / 16384;
-      rps = rps < 1 ? 1 : (rps > 64 ? 64 : rps);
+      int64_t rps = jacobian_strip_height(B);
+      if (p.jac_rps > 0) rps = p.jac_rps; // pda_problem_set_jacobian_strip_height
       const int64_t nstrip = (bh + rps - 1) / rps;
       const View v = generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size()));
       const int64_t nframe = (v.nvisit + 63) / 64;
@@ -587,6 +587,14 @@ int64_t velocity_strip_height(const Box &B) {
   const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
   const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
   int64_t rps = planes > 1 ? bh / 20 : (bh * nseg + 3071) / 3072;
+  return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
+}
+
+// default strip height of the first-order 2D Euler Jacobian march: about
+// 16k one-wave groups (2 rows per strip at 1024^2)
+int64_t jacobian_strip_height(const Box &B) {
+  const int64_t nseg = (B.hi[0] - B.lo[0] + ESEG - 1) / ESEG;
+  const int64_t rps = ((B.hi[1] - B.lo[1]) * nseg) / 16384;
   return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
 }
 

@@ -199,6 +199,7 @@ struct pda_problem_s {
   pda::DevBuf d_vrem_rows;
   int use_box = 1;                   // 0 forces the generic kernels everywhere
   int vel_rps = 0;                   // velocity march strip height (0: from the grid size)
+  int jac_rps = 0;                   // first-order 2D Euler Jacobian strip height (0: from the grid size)
   pda::View view;
   pda::DevBuf d_graph, d_ghost_row, d_near_rows, d_ghost, d_bcf;
   pda::DevBuf d_rowptr, d_colidx, d_rowbase, d_nblk, d_slot, d_bcol;
@@ -227,6 +228,7 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
                         hipStream_t s, bool *rest_done);
 int64_t velocity_march_blocks(const Box &B, int rows_per_strip);
 int64_t velocity_strip_height(const Box &B);
+int64_t jacobian_strip_height(const Box &B);
 int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
                           const double *U, double *V, hipStream_t s);
 // view of the generic kernels: every row (rows == nullptr) or a row list

@@ -719,6 +719,15 @@ int pda_problem_set_velocity_strip_height(pda_problem p, int rows_per_strip) {
   return PDA_OK;
 }
 
+int pda_problem_set_jacobian_strip_height(pda_problem p, int rows_per_strip) {
+  if (!p || rows_per_strip < 0 || rows_per_strip > 64) {
+    set_error("pda_problem_set_jacobian_strip_height: rows_per_strip must be 0..64");
+    return PDA_E_INVALID_ARG;
+  }
+  p->jac_rps = rows_per_strip;
+  return PDA_OK;
+}
+
 int pda_jacobian(pda_problem p, const double *d_U, double t, double *d_J, double *d_V,
                  pda_stream stream) {
   if (!p || !d_U || !d_J) {

@@ -81,6 +81,7 @@ def lib():
             "pda_velocity": ([vp, vp, dbl, vp, vp], i32),
             "pda_problem_autotune": ([vp, vp, vp, P(i32), P(dbl)], i32),
             "pda_problem_set_velocity_strip_height": ([vp, i32], i32),
+            "pda_problem_set_jacobian_strip_height": ([vp, i32], i32),
             "pda_jacobian": ([vp, vp, dbl, vp, vp, vp], i32),
             "pda_apply_jacobian": ([vp, vp, dbl, vp, i64, i32, vp, vp], i32),
             "pda_jacobian_blocks": ([vp, P(i32), P(vp), P(vp)], i32),
@@ -457,6 +458,12 @@ class Problem:
         rule.  The velocity values do not depend on the height."""
         _check(lib().pda_problem_set_velocity_strip_height(self._h, int(rows_per_strip)))
 
+    def set_jacobian_strip_height(self, rows_per_strip):
+        """Fix the strip height of the first-order 2D Euler Jacobian kernel
+        (pda_problem_set_jacobian_strip_height); 0 restores the grid-size
+        rule.  The Jacobian values do not depend on the height."""
+        _check(lib().pda_problem_set_jacobian_strip_height(self._h, int(rows_per_strip)))
+
     rhs = rightHandSide
     __call__ = rightHandSide
 
@@ -691,14 +698,22 @@ class GuessNet:
         self.scales = tuple(float(v) for v in (dt, s_f, s_g, s_o, nu_mean, nu_scale))
         _check(lib().pda_guessnet_set_scales(self._h, *self.scales))
 
+    def _check_fields(self, **fields):
+        n = 2 * self.nx * self.ny
+        for name, t in fields.items():
+            _dev_tensor(t, n, name)
+
     def predict(self, un, unm1, fn, fnm1, nu, out=None):
+        self._check_fields(un=un, unm1=unm1, fn=fn, fnm1=fnm1)
         if out is None:
             out = _torch().empty_like(un)
+        _dev_tensor(out, 2 * self.nx * self.ny, "out")
         _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), float(nu),
                                           _ptr(out), _stream()))
         return out
 
     def gradients(self, un, unm1, fn, fnm1, unp1, nu):
+        self._check_fields(un=un, unm1=unm1, fn=fn, fnm1=fnm1, unp1=unp1)
         g = np.zeros(self.nparams, dtype=np.float32)
         o = np.zeros(self.nx * self.ny * 2, dtype=np.float32)
         loss = ctypes.c_float()
@@ -709,6 +724,7 @@ class GuessNet:
         return loss.value, g, o
 
     def train_step(self, un, unm1, fn, fnm1, unp1, nu, lr, want_loss=False):
+        self._check_fields(un=un, unm1=unm1, fn=fn, fnm1=fnm1, unp1=unp1)
         loss = ctypes.c_float()
         _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), _ptr(unp1),
                                              float(nu), float(lr),
@@ -759,6 +775,7 @@ class Batch:
         _check(lib().pda_batch_velocity(self._h, _ptr(U), float(t), _ptr(V), _stream()))
 
     def autotune(self, U):
+        _dev_tensor(U, self.total, "state")
         r, us = ctypes.c_int(), ctypes.c_double()
         _check(lib().pda_batch_autotune(self._h, _ptr(U), _stream(), ctypes.byref(r), ctypes.byref(us)))
         return r.value, us.value

@@ -99,3 +99,37 @@ def test_training_reduces_loss(torch):
     last = net.train_step(a, b, f0, f1, c, 0.004, 3e-3, want_loss=True)
     print(f"[gpu-nn] loss {first:.3e} -> {last:.3e}", flush=True)
     assert last < 0.1 * first
+
+
+def test_inputs_are_validated_before_any_launch(torch):
+    """predict / gradients / train_step refuse a field of the wrong size,
+    dtype or device (the kernels read and write 2*nx*ny entries of each)."""
+    net = pda.GuessNet(32, 16)
+    n = 2 * 32 * 16
+    good = torch.zeros(n, dtype=torch.float64, device="cuda")
+    bad = [torch.zeros(2 * n, dtype=torch.float64, device="cuda"),     # another grid's state
+           torch.zeros(n, dtype=torch.float32, device="cuda"),         # wrong dtype
+           torch.zeros(n, dtype=torch.float64)]                        # host memory
+    for b in bad:
+        with pytest.raises(pda.PdaError):
+            net.predict(good, good, b, good, 0.005)
+        with pytest.raises(pda.PdaError):
+            net.predict(good, good, good, good, 0.005, out=b)
+        with pytest.raises(pda.PdaError):
+            net.gradients(good, good, good, good, b, 0.005)
+        with pytest.raises(pda.PdaError):
+            net.train_step(b, good, good, good, good, 0.005, 1e-3)
+    out = net.predict(good, good, good, good, 0.005)
+    torch.cuda.synchronize()
+    assert out.shape == good.shape and torch.isfinite(out).all()
+
+
+def test_batch_autotune_validates_state(torch):
+    mesh = pda.create_structured_mesh((64, 64), (-5.0, 5.0, -5.0, 5.0), 7)
+    probs = [pda.create_problem(mesh, pda.Swe2d.SlipWall, pda.InviscidFluxReconstruction.Weno5, 1)
+             for _ in range(2)]
+    b = pda.Batch(probs)
+    with pytest.raises(pda.PdaError):
+        b.autotune(torch.zeros(b.total // 2, dtype=torch.float64, device="cuda"))
+    with pytest.raises(pda.PdaError):
+        b.autotune(torch.zeros(b.total, dtype=torch.float32, device="cuda"))

@@ -1,0 +1,172 @@
+"""The kernel shapes the measured workloads run, against the oracle.
+
+The marching kernels process a strip of grid rows per workgroup; the strip
+height follows the grid size (or a tuner), so small test grids and the
+1024 x 1024 workload take different code paths (the first-order 2D Euler
+Jacobian carries each row's y-face sides to the next row and prefetches the
+next row's states only when a strip has two or more rows).  Here:
+  - the first-order 2D Euler Jacobian at strip heights 1, 2, 3, 8 and the
+    grid-size rule gives bit-identical values, within 1e-12 of the oracle;
+  - the 3D velocity march at heights 1, 4, 12 and the rule likewise;
+  - the 1024 x 1024 Riemann workload itself (WENO5 velocity at the heights
+    the step loop picks from, first-order Jacobian at the grid-size rule of
+    2 rows): pattern bit-exact, values within 1e-12;
+  - velocity and WENO5 / first-order Jacobians on states evolved into shocks
+    by the device stepper, within 1e-12.
+"""
+import numpy as np
+import pytest
+
+import cases
+import pressiodemoapps_amd as pda
+
+pytestmark = pytest.mark.gpu
+S = pda.InviscidFluxReconstruction
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as _t
+    if not _t.cuda.is_available():
+        pytest.skip("no HIP device")
+    return _t
+
+
+def _big(name):
+    return [c for c in cases.BIG_CASES if c[0] == name][0]
+
+
+def _jac(prob, Ud, torch):
+    J = prob.createJacobian()
+    prob.jacobian(Ud, 0.0, J)
+    torch.cuda.synchronize()
+    return J
+
+
+def test_jacobian_strip_heights(oracle, torch):
+    # 150 x 37: box rows 1..35 (35 rows: strips of 8 end with a 3-row strip),
+    # five 31-cell segments per row, the last one partial
+    case = _big("euler2d-riemann-big-FirstOrder")
+    mesh, prob = cases.make(case)
+    ms, pb = oracle.from_mesh(mesh), cases.oracle_problem(case)
+    for seed in (0, 1):
+        U = cases.perturbed_state(prob, seed)
+        Ud = torch.from_numpy(U).cuda()
+        _, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, U)
+        ref = None
+        for h in (1, 2, 3, 8, 0):
+            print(f"[gpu-strips] jacobian 150x37 seed {seed} height {h}", flush=True)
+            prob.set_jacobian_strip_height(h)
+            J = _jac(prob, Ud, torch)
+            if ref is None:
+                ref = J.values.clone()
+                assert np.array_equal(J.rowptr.cpu().numpy(), rpo)
+                assert np.array_equal(J.colidx.cpu().numpy(), cio)
+            else:
+                assert torch.equal(J.values, ref), h
+        err = cases.rel_err(ref.cpu().numpy(), Jo)
+        print(f"[gpu-strips] jacobian 150x37 seed {seed}: {err:.2e}", flush=True)
+        assert err <= TOL, err
+    for bad in (-1, 65):
+        with pytest.raises(pda.PdaError):
+            prob.set_jacobian_strip_height(bad)
+
+
+def test_velocity_march_3d_strip_heights(oracle, torch):
+    # Sedov 3D, WENO5: 40 grid rows per plane, so 12-row strips end with a
+    # 4-row strip; the default rule gives 2 rows here and 12 at 256^3
+    case = ("euler3d-sedov-tall-Weno5", pda.Euler3d.SedovSymmetry, S.Weno5, (24, 40, 9),
+            (0, 1.2, 0, 1.2, 0, 1.2), 7, (), 0, None)
+    mesh, prob = cases.make(case)
+    ms, pb = oracle.from_mesh(mesh), cases.oracle_problem(case)
+    U = cases.perturbed_state(prob, 4)
+    Ud = torch.from_numpy(U).cuda()
+    Vo = oracle.velocity(ms, pb, U)
+    ref = None
+    for h in (1, 4, 12, 0):
+        print(f"[gpu-strips] 3D velocity height {h}", flush=True)
+        prob.set_velocity_strip_height(h)
+        V = prob.createRightHandSide()
+        prob.rightHandSide(Ud, 0.0, V)
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = V.clone()
+        else:
+            assert torch.equal(V, ref), h
+    err = cases.rel_err(ref.cpu().numpy(), Vo)
+    print(f"[gpu-strips] 3D velocity: {err:.2e}", flush=True)
+    assert err <= TOL, err
+
+
+def test_riemann_1024_workload(oracle, torch):
+    """The measured 1024^2 workload: WENO5 velocity (problem A) at the strip
+    heights the step loop chooses from, first-order jacobian() (problem B)
+    at its grid-size rule (2 rows per strip) and at 3 rows."""
+    n = 1024
+    mesh = pda.create_structured_mesh((n, n), (0.0, 1.0, 0.0, 1.0), 7)
+    pa = pda.create_problem(mesh, pda.Euler2d.Riemann, S.Weno5, 2)
+    pb = pda.create_problem(mesh, pda.Euler2d.Riemann, S.FirstOrder, 2)
+    U = cases.perturbed_state(pb, 7)
+    Ud = torch.from_numpy(U).cuda()
+    ms = oracle.from_mesh(mesh)
+    print("[gpu-strips] 1024^2 velocity", flush=True)
+    Vo = oracle.velocity(ms, oracle.problem(2, 4, 2, [1.4]), U)
+    ref = None
+    for h in (2, 3, 4, 6, 0):
+        pa.set_velocity_strip_height(h)
+        V = pa.createRightHandSide()
+        pa.rightHandSide(Ud, 0.0, V)
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = V.clone()
+        else:
+            assert torch.equal(V, ref), h
+    err = cases.rel_err(ref.cpu().numpy(), Vo)
+    print(f"[gpu-strips] 1024^2 velocity: {err:.2e}", flush=True)
+    assert err <= TOL, err
+    del Vo, ref
+    print("[gpu-strips] 1024^2 first-order jacobian", flush=True)
+    _, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, oracle.problem(2, 4, 0, [1.4]), U)
+    J = _jac(pb, Ud, torch)
+    assert np.array_equal(J.rowptr.cpu().numpy(), rpo), "row pointers differ"
+    assert np.array_equal(J.colidx.cpu().numpy(), cio), "column indices differ"
+    del rpo, cio
+    err = cases.rel_err(J.values.cpu().numpy(), Jo)
+    print(f"[gpu-strips] 1024^2 jacobian (rule): {err:.2e}", flush=True)
+    assert err <= TOL, err
+    pb.set_jacobian_strip_height(3)
+    J3 = _jac(pb, Ud, torch)
+    assert torch.equal(J3.values, J.values)
+
+
+@pytest.mark.parametrize("scheme", [S.Weno5, S.FirstOrder], ids=lambda s: s.name)
+def test_jacobian_on_evolved_shock_state(scheme, oracle, torch):
+    """Riemann IC2 on 64 x 64 advanced by the device SSPRK3 stepper to
+    t = 0.3 (shocks and contacts inside the grid), then velocity, velocity
+    from the Jacobian path and Jacobian values against the oracle on that
+    state."""
+    mesh = pda.create_structured_mesh((64, 64), (0.0, 1.0, 0.0, 1.0), 7)
+    prob = pda.create_problem(mesh, pda.Euler2d.Riemann, scheme, 2)
+    U = prob.initialCondition()
+    pda.advanceSSP3(prob, U, 2e-3, 150)
+    torch.cuda.synchronize()
+    Uh = U.cpu().numpy()
+    assert np.isfinite(Uh).all()
+    rho = Uh[0::4]
+    assert rho.max() / rho.min() > 5.0  # strong gradients are present
+    V = prob.createRightHandSide()
+    prob.rightHandSide(U, 0.0, V)
+    V2 = prob.createRightHandSide()
+    J = prob.createJacobian()
+    prob.rightHandSideAndJacobian(U, 0.0, V2, J)
+    torch.cuda.synchronize()
+    ms = oracle.from_mesh(mesh)
+    pb = oracle.problem(2, 4, int(scheme), [1.4])
+    Vo = oracle.velocity(ms, pb, Uh)
+    V2o, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, Uh)
+    e = (cases.rel_err(V.cpu().numpy(), Vo), cases.rel_err(V2.cpu().numpy(), V2o),
+         cases.rel_err(J.values.cpu().numpy(), Jo))
+    print(f"[gpu-strips] evolved Riemann {scheme.name}: V {e[0]:.2e} V(J) {e[1]:.2e} J {e[2]:.2e}", flush=True)
+    assert np.array_equal(J.colidx.cpu().numpy(), cio)
+    assert max(e) <= TOL, e

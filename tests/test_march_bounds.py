@@ -1,0 +1,208 @@
+"""CPU bounds model of the velocity march (vel_march_body in
+csrc/pda_box_kernels.hip), the kernel that once read past the grid.
+
+The model restates the kernel's index arithmetic (workgroup -> plane, strip,
+segment; lane -> column; the strip's row loop; the face stencils and their
+mapping along the face axis; the source-term reads; the velocity stores),
+vectorized over workgroups, lanes and rows, and checks for every box shape
+the library builds (the interior box of each scheme width, and the full grid
+with periodic or mirror-ghost axes), every strip height 1..64 and grids from
+the smallest allowed upwards that
+  - every state read is a cell inside the grid after the axis mapping;
+  - every velocity store is a box cell, and every box cell is stored exactly
+    once;
+  - in a batch, the workgroup -> instance table gives every workgroup a
+    local index inside its instance and instance slices do not overlap.
+It also checks that it flags the pre-fix indexing (lanes past the last cell
+evaluating y faces and sources at column nx instead of nx - 1), so it is a
+regression guard for that fault: the ghost rule the march reads by index is
+the mirror image about the boundary (impl/euler_2d_ghost_filler_neumann.hpp:
+163-205).
+"""
+import itertools
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "pressio-demoapps_amd", "csrc", "pda_box_kernels.hip")
+VSEG = 63
+
+
+def test_segment_width_matches_kernel_source():
+    txt = open(SRC).read()
+    assert re.search(r"constexpr int VSEG = (\d+);", txt).group(1) == str(VSEG)
+    # the kernel maps lanes past the last cell to column nx - 1 for every
+    # read that is not along x
+    assert "const int64_t ic = i < nx ? i : nx - 1;" in txt
+
+
+def _map(c, n, mode):
+    """Index along the face axis after the kernel's mapping, and whether the
+    read is legal: mode 0 never maps (the box keeps stencils inside the
+    grid), 1 wraps periodically, 2 mirrors about the boundary."""
+    if mode == 0:
+        return c, (c >= 0) & (c < n)
+    if mode == 1:
+        m = np.where(c < 0, c + n, np.where(c >= n, c - n, c))
+    else:
+        m = np.where(c < 0, -1 - c, np.where(c >= n, 2 * n - 1 - c, c))
+    return m, (m >= 0) & (m < n)
+
+
+def march_accesses(box, W, rps, clamp=True):
+    """All (reads, stores) of one velocity launch over `box`.
+
+    box = (lo, hi, n, mode), 3-tuples; returns
+      bad_reads: number of state reads outside the grid,
+      stores: flat cell ids of the velocity stores (with repeats)."""
+    lo, hi, n, mode = box
+    dim = 3 if n[2] > 1 else 2
+    bw, bh, planes = hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]
+    nseg = -(-bw // VSEG)
+    nstrip = -(-bh // rps)
+    bid = np.arange(nseg * nstrip * planes)
+    plane = bid // (nseg * nstrip)
+    rem = bid - plane * nseg * nstrip
+    seg, strip = rem % nseg, rem // nseg
+    kz = lo[2] + plane
+    iseg = lo[0] + seg * VSEG
+    ncell = np.minimum(bw - seg * VSEG, VSEG)
+    j0 = lo[1] + strip * rps
+    j1 = np.minimum(j0 + rps, hi[1])
+    lane = np.arange(64)
+    # (block, lane, row iteration)
+    i = (iseg[:, None] + np.minimum(lane[None, :], ncell[:, None]))[:, :, None]
+    ic = np.minimum(i, n[0] - 1) if clamp else i
+    own = (lane[None, :] < ncell[:, None])[:, :, None]
+    it = np.arange(rps + 1)
+    j = ((j0 - 1)[:, None] + it[None, :])[:, None, :]        # j0-1 .. j0+rps-1
+    live = j < j1[:, None, None]                             # loop condition
+    body = live & (j >= j0[:, None, None])                   # past the `continue`
+    k = kz[:, None, None]
+    shape = np.broadcast_shapes(i.shape, j.shape)
+    I, IC, J, K = (np.broadcast_to(a, shape) for a in (i, ic, j, k))
+    LIVE, BODY = np.broadcast_to(live, shape), np.broadcast_to(body, shape)
+    bad = 0
+
+    def check(ci, cj, ck, mask, axis):
+        nonlocal bad
+        coords = [ci, cj, ck]
+        m, ok = _map(coords[axis], n[axis], mode[axis])
+        coords[axis] = m
+        for a in range(3):
+            if a != axis:
+                ok = ok & (coords[a] >= 0) & (coords[a] < n[a])
+        bad += int(np.count_nonzero(mask & ~ok))
+
+    for p in range(2 * W):
+        check(IC, J + 1 + p - W, K, LIVE, 1)       # y face above the row
+        check(I + p - W, J, K, BODY, 0)            # x face on the left of column i
+        if dim == 3:
+            check(IC, J, K + p - W, BODY, 2)       # z face below
+            check(IC, J, K + 1 + p - W, BODY, 2)   # z face above
+    # source terms read the cell itself (and, for Burgers, its graph row)
+    ok = (IC >= 0) & (IC < n[0]) & (J >= 0) & (J < n[1]) & (K >= 0) & (K < n[2])
+    bad += int(np.count_nonzero(BODY & ~ok))
+    st = BODY & np.broadcast_to(own, shape)
+    stores = ((K * n[1] + J) * n[0] + IC)[st]
+    return bad, stores
+
+
+def boxes(n, W):
+    """Every box the library builds for a grid: the interior box [W, n-W),
+    and full-grid axes (periodic or mirror) where the grid is at least
+    2W+1 cells wide (pda_problem.cpp's velocity box)."""
+    dim = 3 if n[2] > 1 else 2
+    opts = []
+    for a in range(3):
+        if a >= dim:
+            opts.append([(0, 1, 0)])
+            continue
+        o = []
+        if n[a] - W > W:
+            o.append((W, n[a] - W, 0))
+        if n[a] >= 2 * W + 1:
+            o += [(0, n[a], 1), (0, n[a], 2)]
+        opts.append(o)
+    for combo in itertools.product(*opts):
+        lo = tuple(c[0] for c in combo)
+        hi = tuple(c[1] for c in combo)
+        mode = tuple(c[2] for c in combo)
+        yield lo, hi, tuple(n), mode
+
+
+GRIDS_2D = [(7, 7, 1), (13, 5, 1), (63, 9, 1), (64, 12, 1), (70, 131, 1), (127, 7, 1)]
+GRIDS_3D = [(7, 7, 7), (10, 9, 8), (24, 40, 9), (64, 13, 5)]
+
+
+@pytest.mark.parametrize("grid", GRIDS_2D + GRIDS_3D, ids=lambda g: "x".join(map(str, g)))
+def test_march_stays_in_grid(grid):
+    heights = range(1, 65) if grid[2] == 1 else (1, 2, 3, 4, 5, 7, 12, 16, 33, 64)
+    nbox = 0
+    for W in (1, 2, 3):
+        for box in boxes(grid, W):
+            lo, hi, n, mode = box
+            cells = np.array([(k * n[1] + j) * n[0] + i for k in range(lo[2], hi[2])
+                              for j in range(lo[1], hi[1]) for i in range(lo[0], hi[0])])
+            for rps in heights:
+                bad, stores = march_accesses(box, W, rps)
+                assert bad == 0, (box, W, rps, bad)
+                # every box cell is stored exactly once
+                assert len(stores) == len(cells), (box, W, rps)
+                assert np.array_equal(np.sort(stores), np.sort(cells)), (box, W, rps)
+            nbox += 1
+    assert nbox > 0
+
+
+def test_model_flags_the_pre_fix_indexing():
+    """Before the fix, lanes past the last cell read y faces and sources at
+    column nx: outside the grid whenever the box spans the whole x axis."""
+    for grid in ((70, 131, 1), (64, 12, 1), (10, 9, 8)):
+        flagged = 0
+        for W in (1, 2, 3):
+            for box in boxes(grid, W):
+                bad, _ = march_accesses(box, W, 3, clamp=False)
+                if box[3][0] != 0:
+                    assert bad > 0, (box, W)
+                    flagged += 1
+                else:
+                    assert bad == 0, (box, W)
+        assert flagged > 0
+
+
+def velocity_march_blocks(box, rps):
+    lo, hi, n, mode = box
+    return -(-(hi[0] - lo[0]) // VSEG) * -(-(hi[1] - lo[1]) // rps) * (hi[2] - lo[2])
+
+
+def test_batch_instance_table():
+    """The batch tables (pda_batch.hip build_tables): consecutive workgroup
+    ranges per instance, each instance's state slice after the previous one;
+    every workgroup's local index addresses only its own instance."""
+    rng = np.random.default_rng(3)
+    sizes = [int(x) for x in rng.choice([64, 96, 128, 192, 256], size=24)]
+    m, W = 3, 3
+    for rps in (1, 2, 5, 8, 12, 32, 64):
+        inst, blk_inst, off = [], [], 0
+        blk = 0
+        for q, s in enumerate(sizes):
+            box = ((0, 0, 0), (s, s, 1), (s, s, 1), (2, 2, 0))
+            nb = velocity_march_blocks(box, rps)
+            inst.append((box, off, blk, nb))
+            blk_inst += [q] * nb
+            blk += nb
+            off += s * s * m
+        total = off
+        blk_inst = np.array(blk_inst)
+        for b in range(0, len(blk_inst), 7):
+            box, o, b0, nb = inst[blk_inst[b]]
+            assert 0 <= b - b0 < nb
+        for q, (box, o, b0, nb) in enumerate(inst):
+            end = o + box[2][0] * box[2][1] * m
+            assert end <= total and (q + 1 == len(inst) or inst[q + 1][1] == end)
+            bad, stores = march_accesses(box, W, rps)
+            assert bad == 0
+            assert stores.max() * m + m <= end - o
