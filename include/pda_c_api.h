@@ -170,6 +170,24 @@ int pda_apply_jacobian(pda_problem p, const double *d_U, double t,
                        const double *d_B, int64_t ncols, int layout,
                        double *d_AJ, pda_stream stream);
 
+/* applyJacobian for callers that cannot allocate device memory.  No
+   counterpart in the reference: it is for code under HIP graph capture,
+   tight serving loops and harnesses that forbid hipMalloc.  The same
+   computation as pda_apply_jacobian, with the Jacobian values held in one
+   fixed device array of PDA_NOALLOC_MAX_NNZ doubles (64 MiB, reserved on
+   every device the library runs on until the process ends) instead of a
+   per-problem buffer allocated on first use; it calls no allocating
+   routine.  Refuses (PDA_E_INVALID_ARG, before any device work) a problem
+   with more nonzeros than the array holds, and (PDA_E_STATE) a problem
+   whose device tables were never built: call pda_jacobian_pattern or any
+   evaluation once before.  All calls on one device share the array, so
+   calls that overlap on one device (other threads, other streams) race;
+   calls ordered on one stream do not. */
+#define PDA_NOALLOC_MAX_NNZ 8388608
+int pda_apply_jacobian_noalloc(pda_problem p, const double *d_U, double t,
+                               const double *d_B, int64_t ncols, int layout,
+                               double *d_AJ, pda_stream stream);
+
 /* ---------------------------------------------------- explicit stepping */
 
 /* Explicit time integration of dU/dt = velocity(U) on the device with

@@ -29,7 +29,7 @@ COMMON = [
 
 SOURCES = ["pda_mesh.cpp", "pda_problem.cpp", "pda_kernels.hip", "pda_box_kernels.hip",
            "pda_stepper.hip", "pda_implicit.hip",
-           "pda_guessnet.hip", "pda_batch.hip"]
+           "pda_guessnet.hip", "pda_batch.hip", "pda_noalloc.hip"]
 
 
 def _compile(src):
