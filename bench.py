@@ -163,52 +163,86 @@ def extra_workloads():
     return out
 
 
-def cpu_baseline(n_cpu, budget_s):
-    """Reference CPU path (oracle/_ref, OpenMP) or the C restatement, timed on
-    this host on an n_cpu x n_cpu sample of the same workload."""
+def cpu_baseline_child(meshdir, reps):
+    """Runs in a child process that never imports torch (so the reference's
+    OpenMP runtime starts with this process's OMP_* settings): the
+    reference's own rightHandSide (WENO5, problem A), jacobian and
+    rightHandSideAndJacobian (first order, problem B) on a state, velocity
+    and Jacobian created once, as a caller of adapter_cpp.hpp:105-229 runs
+    them; one warm-up call then `reps` timed calls each.  Prints one JSON
+    line with the per-call times."""
+    import numpy as np
+    import pyoracle
+    path, level = pyoracle.ref_omp_build()
+    t0 = time.perf_counter()
+    ra = pyoracle.RefProblem(meshdir, 2, 4, 2, 2, omp=True)
+    rb = pyoracle.RefProblem(meshdir, 2, 4, 0, 2, omp=True)
+    setup = time.perf_counter() - t0
+    U0 = ra.initial_condition()
+    U = U0 * (1.0 + 0.01 * np.random.default_rng(0).uniform(-1, 1, U0.shape))
+    out = {"rhs": ra.time_calls(U, 0, reps).tolist(), "jacobian": rb.time_calls(U, 1, reps).tolist(),
+           "rhs_and_jacobian": rb.time_calls(U, 2, reps).tolist(), "setup_s": setup,
+           "build": os.path.basename(path), "isa": level}
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(n_cpu, reps=5):
+    """The reference CPU path (oracle/_ref, OpenMP, the fastest x86-64 level
+    this host runs) timed on this host's cores on the n_cpu x n_cpu Riemann
+    workload: cells / (median rightHandSide + median jacobian) per
+    perturbation.  Without the reference build, the C restatement (one
+    thread) stands in."""
     import numpy as np
     import pyoracle
     import pressiodemoapps_amd as pda
     mesh = pda.create_structured_mesh((n_cpu, n_cpu), (0.0, 1.0, 0.0, 1.0), 7)
     cells = n_cpu * n_cpu
-    rng = np.random.default_rng(0)
-    threads = os.cpu_count() or 1
     if pyoracle.ref_available(omp=True):
+        import subprocess
         d = tempfile.mkdtemp(prefix="pda_bench_mesh_")
         mesh.write(d)
-        ra = pyoracle.RefProblem(d, 2, 4, 2, 2, omp=True)
-        rb = pyoracle.RefProblem(d, 2, 4, 0, 2, omp=True)
-        U0 = ra.initial_condition()
-        kind = "reference"
-        velocity = lambda U: ra.velocity(U)  # noqa: E731
-        jacobian = lambda U: rb.velocity_and_jacobian(U)  # noqa: E731
-        threads = int(os.environ.get("OMP_NUM_THREADS", threads))
-    else:
-        orc = pyoracle.Oracle()
-        ms = orc.from_mesh(mesh)
-        import ctypes
-        pa = pda.create_problem(mesh, pda.Euler2d.Riemann, pda.InviscidFluxReconstruction.Weno5, 2)
-        U0 = np.zeros(pa.totalDofStencilMesh())
-        pda.lib().pda_initial_condition_host(pa.handle, U0.ctypes.data_as(ctypes.c_void_p))
-        pba, pbb = orc.problem(2, 4, 2, [1.4]), orc.problem(2, 4, 0, [1.4])
-        kind = "port"
-        threads = 1
-        velocity = lambda U: orc.velocity(ms, pba, U)  # noqa: E731
-        jacobian = lambda U: orc.velocity_and_jacobian(ms, pbb, U)  # noqa: E731
-    U = U0 * (1.0 + 0.01 * rng.uniform(-1, 1, U0.shape))
-    velocity(U)
-    jacobian(U)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        velocity(U)
-        jacobian(U)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s or reps >= 200:
-            break
-    return {"value": reps * cells / el, "unit": "cells/s", "cores": threads, "kind": kind,
-            "sample": f"{n_cpu}x{n_cpu} Riemann IC2 (same mesh/scheme), {reps} perturbations of "
-                      f"WENO5 rightHandSide + first-order jacobian in {el:.1f} s, {threads} threads"}
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="true")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-child", d, "--cpu-reps", str(reps)],
+                           env=env, capture_output=True, text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError("reference timing failed: " + r.stderr[-2000:])
+        t = json.loads(r.stdout.strip().splitlines()[-1])
+        med = {k: float(np.median(t[k])) for k in ("rhs", "jacobian", "rhs_and_jacobian")}
+        per = med["rhs"] + med["jacobian"]
+        return {"value": cells / per, "unit": "cells/s", "cores": threads, "kind": "reference",
+                "sample": f"{n_cpu}x{n_cpu} Riemann IC2, the full workload size: median of {reps} calls "
+                          f"(after a warm-up) of the reference's WENO5 rightHandSide "
+                          f"{med['rhs'] * 1e3:.1f} ms + first-order jacobian {med['jacobian'] * 1e3:.1f} ms "
+                          f"per perturbation, V and J created once; {threads} OpenMP threads, "
+                          f"{t['isa']} build",
+                "rhs_ms": med["rhs"] * 1e3, "jacobian_ms": med["jacobian"] * 1e3,
+                "rhs_and_jacobian_ms": med["rhs_and_jacobian"] * 1e3,
+                "rhs_cells_per_s": cells / med["rhs"], "jacobian_cells_per_s": cells / med["jacobian"],
+                "setup_s": t["setup_s"], "build": t["build"]}
+    orc = pyoracle.Oracle()
+    ms = orc.from_mesh(mesh)
+    import ctypes
+    pa = pda.create_problem(mesh, pda.Euler2d.Riemann, pda.InviscidFluxReconstruction.Weno5, 2)
+    U0 = np.zeros(pa.totalDofStencilMesh())
+    pda.lib().pda_initial_condition_host(pa.handle, U0.ctypes.data_as(ctypes.c_void_p))
+    U = U0 * (1.0 + 0.01 * np.random.default_rng(0).uniform(-1, 1, U0.shape))
+    pba, pbb = orc.problem(2, 4, 2, [1.4]), orc.problem(2, 4, 0, [1.4])
+    tv, tj = [], []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        orc.velocity(ms, pba, U)
+        t1 = time.perf_counter()
+        orc.velocity_and_jacobian(ms, pbb, U)
+        t2 = time.perf_counter()
+        if r:
+            tv.append(t1 - t0)
+            tj.append(t2 - t1)
+    per = float(np.median(tv) + np.median(tj))
+    return {"value": cells / per, "unit": "cells/s", "cores": 1, "kind": "port",
+            "sample": f"{n_cpu}x{n_cpu} Riemann IC2: median of {reps} calls of the C restatement's WENO5 "
+                      f"velocity + first-order Jacobian, one thread",
+            "rhs_ms": float(np.median(tv)) * 1e3, "jacobian_ms": float(np.median(tj)) * 1e3}
 
 
 def main():
@@ -217,8 +251,9 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--cpu-n", type=int, default=512)
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-n", type=int, default=1024)
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--cpu-child", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched shallow-water and implicit Burgers measurements")
@@ -226,6 +261,9 @@ def main():
                     help="after timing, run one plain 256 MiB device copy (the reference "
                          "kernel for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    if args.cpu_child:
+        cpu_baseline_child(args.cpu_child, args.cpu_reps)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU is measured here; --gpus must be 1 (several GPUs would "
                                    "each own a slab of the mesh with its halo rows)"}))
@@ -354,7 +392,7 @@ def main():
     if not args.no_cpu:
         log("[bench] cpu baseline")
         try:
-            cpu = cpu_baseline(args.cpu_n, args.cpu_budget)
+            cpu = cpu_baseline(args.cpu_n, args.cpu_reps)
         except Exception as e:  # reported, never fatal
             log(f"[bench] cpu baseline failed: {e}")
     out = {

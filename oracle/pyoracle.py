@@ -167,8 +167,35 @@ class Oracle:
         return JL.reshape(m, m), JR.reshape(m, m)
 
 
+def _cpu_flags():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    return set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return set()
+
+
+# x86-64 micro-architecture levels (LZCNT is listed as "abm")
+_V3 = {"avx", "avx2", "bmi1", "bmi2", "f16c", "fma", "abm", "movbe"}
+_V4 = _V3 | {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"}
+
+
+def ref_omp_build():
+    """(path, level) of the fastest OpenMP reference build this host can
+    run: x86-64-v4 (AVX-512), v3 (AVX2 + FMA) or the baseline x86-64."""
+    flags = _cpu_flags()
+    for level, need in (("v4", _V4), ("v3", _V3)):
+        path = os.path.join(HERE, "_ref", f"libpda_ref_omp_{level}.so")
+        if need <= flags and os.path.exists(path):
+            return path, "x86-64-" + level
+    return REF_OMP_SO, "x86-64"
+
+
 def ref_available(omp=False):
-    return os.path.exists(REF_OMP_SO if omp else REF_SO)
+    return os.path.exists(ref_omp_build()[0] if omp else REF_SO)
 
 
 class RefProblem:
@@ -176,7 +203,7 @@ class RefProblem:
     _libs = {}
 
     def __init__(self, meshdir, family, prob, scheme, icflag=0, params=None, omp=False):
-        path = REF_OMP_SO if omp else REF_SO
+        path = ref_omp_build()[0] if omp else REF_SO
         if path not in RefProblem._libs:
             L = ctypes.CDLL(path)
             L.ref_create.restype = _vp
@@ -188,6 +215,7 @@ class RefProblem:
             L.ref_velocity.argtypes = [_vp, _P(_dbl), _dbl, _P(_dbl)]
             L.ref_velocity_and_jacobian.argtypes = [_vp, _P(_dbl), _dbl, _P(_dbl), _P(_i64),
                                                     _P(ctypes.c_int32), _P(_dbl)]
+            L.ref_time_calls.argtypes = [_vp, _P(_dbl), _dbl, _i32, _i32, _P(_dbl)]
             L.ref_last_error.restype = ctypes.c_char_p
             RefProblem._libs[path] = L
         self.L = RefProblem._libs[path]
@@ -221,6 +249,16 @@ class RefProblem:
         if self.L.ref_velocity(self.h, _dp(U), t, _dp(V)):
             raise RuntimeError(self.L.ref_last_error().decode())
         return V
+
+    def time_calls(self, U, which, reps, t=0.0):
+        """Wall times (s) of `reps` calls of the reference's rightHandSide
+        (which = 0), jacobian (1) or rightHandSideAndJacobian (2) on a state,
+        velocity and Jacobian created once, after one warm-up call."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        secs = np.zeros(reps)
+        if self.L.ref_time_calls(self.h, _dp(U), t, which, reps, _dp(secs)):
+            raise RuntimeError(self.L.ref_last_error().decode())
+        return secs
 
     def velocity_and_jacobian(self, U, t=0.0):
         U = np.ascontiguousarray(U, dtype=np.float64)

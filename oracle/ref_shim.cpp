@@ -10,12 +10,15 @@
 // (info.dat / coordinates.dat / connectivity.dat), a problem family and the
 // reference enum values, and expose the reference's own
 // rightHandSide / rightHandSideAndJacobian / initialCondition.
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <vector>
 
 #include "pressiodemoapps/euler1d.hpp"
 #include "pressiodemoapps/euler2d.hpp"
@@ -44,6 +47,7 @@ struct Base {
   virtual void ic(double *) const = 0;
   virtual void velocity(const double *, double, double *) const = 0;
   virtual void jacobian(const double *, double, double *, int64_t *, int32_t *, double *) const = 0;
+  virtual void time_calls(const double *, double, int, int, double *) const = 0;
 };
 
 template <class P>
@@ -81,7 +85,30 @@ struct Holder : Base {
       vals[i] = J.valuePtr()[i];
     }
   }
+  void time_calls(const double *U, double t, int which, int reps, double *secs) const override;
 };
+
+template <class P>
+void Holder<P>::time_calls(const double *U, double t, int which, int reps, double *secs) const {
+  // the caller's side of adapter_cpp.hpp:105-229: state, velocity and
+  // Jacobian created once, then the evaluation called on them repeatedly
+  typename P::state_type u(cols());
+  std::memcpy(u.data(), U, sizeof(double) * cols());
+  auto v = p.createRightHandSide();
+  // (no Jacobian copy when only the velocity is timed)
+  auto J = which == 0 ? typename P::jacobian_type() : p.createJacobian();
+  auto call = [&] {
+    if (which == 0) p.rightHandSide(u, t, v);
+    else if (which == 1) p.jacobian(u, t, J);
+    else p.rightHandSideAndJacobian(u, t, v, J);
+  };
+  call();  // warm-up
+  for (int r = 0; r < reps; ++r) {
+    const auto t0 = std::chrono::steady_clock::now();
+    call();
+    secs[r] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+}
 
 template <class P, class F>
 Base *make(std::unique_ptr<mesh_t> m, F &&f) {
@@ -191,6 +218,21 @@ int ref_initial_condition(void *h, double *U) {
 int ref_velocity(void *h, const double *U, double t, double *V) {
   try {
     static_cast<Base *>(h)->velocity(U, t, V);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Times the reference's own evaluation calls as a caller of
+// adapter_cpp.hpp runs them: state, velocity and Jacobian created once,
+// then which = 0 rightHandSide (:193-199), 1 jacobian (:223-229) or
+// 2 rightHandSideAndJacobian (:201-208) called once to warm up and `reps`
+// more times; secs[r] receives the wall time of call r.
+int ref_time_calls(void *h, const double *U, double t, int which, int reps, double *secs) {
+  try {
+    static_cast<Base *>(h)->time_calls(U, t, which, reps, secs);
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();
