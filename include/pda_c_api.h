@@ -198,7 +198,10 @@ int pda_apply_jacobian_noalloc(pda_problem p, const double *d_U, double t,
    (pressiodemoapps/__init__.py:117-185).  Full meshes only; a stepper
    must not outlive its problem. */
 enum { PDA_FORWARD_EULER = 0, PDA_SSPRK3 = 1, PDA_RK4 = 2,
-       PDA_RK2 = 3 /* advanceRK2, pressiodemoapps/__init__.py:90-114 */ };
+       /* the Python module's integrators with its own stage algebra, rounded
+          as numpy rounds it: advanceRK2 (pressiodemoapps/__init__.py:90-114),
+          advanceRK4 (:117-152), advanceSSP3 (:155-185) */
+       PDA_RK2 = 3, PDA_MODULE_RK4 = 4, PDA_MODULE_SSP3 = 5 };
 typedef struct pda_stepper_s *pda_stepper;
 int pda_stepper_create(pda_problem p, int scheme, pda_stepper *out);
 /* nsteps steps of size dt from time t0, in place on d_U.  use_graph != 0

@@ -216,7 +216,7 @@ int pda_batch_set_velocity_strip_height(pda_batch b, int rows_per_strip) {
 
 int pda_batch_advance(pda_batch b, int scheme, double *d_U, double t0, double dt, int64_t nsteps,
                       int steps_per_graph, pda_stream stream) {
-  if (!b || !d_U || nsteps < 0 || scheme < PDA_FORWARD_EULER || scheme > PDA_RK2) {
+  if (!b || !d_U || nsteps < 0 || scheme < PDA_FORWARD_EULER || scheme > PDA_MODULE_SSP3) {
     set_error("pda_batch_advance: invalid argument");
     return PDA_E_INVALID_ARG;
   }

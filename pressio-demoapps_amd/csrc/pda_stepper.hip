@@ -33,7 +33,111 @@ __global__ void __launch_bounds__(256) k_lincomb(double *out, Lin L, int64_t len
   }
 }
 
+// The Python module's stage updates (pressiodemoapps/__init__.py:90-185)
+// as numpy evaluates them: one rounding per product and per sum, in the
+// module's order, no fused multiply-add.  v is the velocity just
+// evaluated; k_i = dt * v_i as the module forms them.
+#pragma clang fp contract(off)
+
+// k = dt * v (in place, kept for the final combination); out = x + a * k
+__global__ void __launch_bounds__(256) k_py_stage(double *kv, const double *x, double *out, double dt, double a,
+                                                  int64_t n) {
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double k = dt * kv[i];
+    kv[i] = k;
+    out[i] = x[i] + a * k;
+  }
+}
+
+// advanceSSP3, second stage: out = 3/4 y + 1/4 y1 + (1/4 dt) v
+__global__ void __launch_bounds__(256) k_py_ssp3_second(double *out, const double *y, const double *y1,
+                                                        const double *v, double qdt, int64_t n) {
+  const double threeOverFour = 3. / 4., oneOverFour = 1. / 4.;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double a = threeOverFour * y[i];
+    const double b = oneOverFour * y1[i];
+    const double c = qdt * v[i];
+    out[i] = (a + b) + c;
+  }
+}
+
+// advanceSSP3, last stage: y = 1/3 (y + 2 y2 + (2 dt) v)
+__global__ void __launch_bounds__(256) k_py_ssp3_final(double *y, const double *y2, const double *v, double tdt,
+                                                       int64_t n) {
+  const double oneOverThree = 1. / 3., two = 2.;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double a = y[i] + two * y2[i];
+    const double b = tdt * v[i];
+    y[i] = oneOverThree * (a + b);
+  }
+}
+
+// advanceRK4, last stage: k4 = dt v; y = y + (k1 + 2 k2 + 2 k3 + k4) / 6
+__global__ void __launch_bounds__(256) k_py_rk4_final(double *y, const double *k1, const double *k2,
+                                                      const double *k3, const double *v, double dt, int64_t n) {
+  const double two = 2., oneOverSix = 1. / 6.;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double k4 = dt * v[i];
+    double sum = k1[i] + two * k2[i];
+    sum = sum + two * k3[i];
+    sum = sum + k4;
+    y[i] = y[i] + sum * oneOverSix;
+  }
+}
+
+// advanceRK2, last stage: k2 = dt v; y = y + k2/2 + k1/2
+__global__ void __launch_bounds__(256) k_py_rk2_final(double *y, const double *k1, const double *v, double dt,
+                                                      int64_t n) {
+  const double half = 0.5;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const double k2 = dt * v[i];
+    const double a = y[i] + k2 * half;
+    y[i] = a + k1[i] * half;
+  }
+}
+
+#pragma clang fp contract(on)
+
+dim3 vec_grid(int64_t len) {
+  return dim3(unsigned(std::max<int64_t>(std::min<int64_t>((len + 255) / 256, 4096), 1)));
+}
+
 }  // namespace
+
+int py_stage(double *kv, const double *x, double *out, double dt, double a, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_py_stage, vec_grid(n), dim3(256), 0, s, kv, x, out, dt, a, n);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+int py_ssp3_second(double *out, const double *y, const double *y1, const double *v, double dt, int64_t n,
+                   hipStream_t s) {
+  const double oneOverFour = 1. / 4.;
+  hipLaunchKernelGGL(k_py_ssp3_second, vec_grid(n), dim3(256), 0, s, out, y, y1, v, oneOverFour * dt, n);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+int py_ssp3_final(double *y, const double *y2, const double *v, double dt, int64_t n, hipStream_t s) {
+  const double two = 2.;
+  hipLaunchKernelGGL(k_py_ssp3_final, vec_grid(n), dim3(256), 0, s, y, y2, v, two * dt, n);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+int py_rk4_final(double *y, const double *k1, const double *k2, const double *k3, const double *v, double dt,
+                 int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_py_rk4_final, vec_grid(n), dim3(256), 0, s, y, k1, k2, k3, v, dt, n);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+int py_rk2_final(double *y, const double *k1, const double *v, double dt, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_py_rk2_final, vec_grid(n), dim3(256), 0, s, y, k1, v, dt, n);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
 
 int lincomb(double *out, std::initializer_list<std::pair<double, const double *>> terms, int64_t len,
             hipStream_t s) {
@@ -124,7 +228,7 @@ int step_once(pda_stepper_s &st, double *U, double t, double dt, hipStream_t s) 
 extern "C" {
 
 int pda_stepper_create(pda_problem p, int scheme, pda_stepper *out) {
-  if (!p || !out || scheme < PDA_FORWARD_EULER || scheme > PDA_RK2) {
+  if (!p || !out || scheme < PDA_FORWARD_EULER || scheme > PDA_MODULE_SSP3) {
     set_error("pda_stepper_create: invalid argument");
     return PDA_E_INVALID_ARG;
   }

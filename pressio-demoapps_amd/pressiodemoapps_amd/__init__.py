@@ -185,8 +185,13 @@ class CellCenteredUniformMesh:
     """Cell-centred uniform mesh (impl/mesh_ccu.hpp:60-473)."""
 
     def __init__(self, handle):
-        self._h = ctypes.c_void_p(handle)
         L = lib()
+        if isinstance(handle, (str, os.PathLike)):
+            # CellCenteredUniformMesh(path), src_py/main_binder.cc: read the directory
+            h = ctypes.c_void_p()
+            _check(L.pda_mesh_load(str(handle).encode(), ctypes.byref(h)))
+            handle = h.value
+        self._h = ctypes.c_void_p(handle)
         dim, st = ctypes.c_int(), ctypes.c_int()
         ns, nst = ctypes.c_int64(), ctypes.c_int64()
         d = (ctypes.c_double * 3)()
@@ -520,9 +525,16 @@ class Problem:
 
 def create_problem(mesh, problem_enum, scheme=InviscidFluxReconstruction.FirstOrder,
                    ic_flag=0, params=None):
-    """create_problem(mesh, ProblemEnum, InviscidFluxReconstruction[, icFlag][, params])."""
+    """create_problem(mesh, ProblemEnum, InviscidFluxReconstruction[, icFlag][, params]);
+    for the Burgers problems, as the reference's binding
+    (src_py/main_binder.cc:387-392), the fourth argument may be the viscous
+    scheme, ViscousFluxReconstruction.FirstOrder."""
     if isinstance(ic_flag, dict):
         params, ic_flag = ic_flag, 0
+    if isinstance(ic_flag, ViscousFluxReconstruction):
+        if not isinstance(problem_enum, AdvectionDiffusion2d):
+            raise PdaError("create_problem: a viscous scheme applies to the Burgers problems only")
+        ic_flag = 0
     return Problem(mesh, problem_enum, scheme, ic_flag, params)
 
 
@@ -550,10 +562,15 @@ def create_burgers_2d_problem(mesh, problem_enum, scheme, viscous_scheme, params
 
 # ------------------------------------------------------- explicit stepping
 class StepScheme(enum.IntEnum):
+    """pressio's stage formulas (ForwardEuler, SSPRK3, RK4) and the module's
+    own integrators with their algebra (RK2, ModuleRK4, ModuleSSP3:
+    pressiodemoapps/__init__.py:90-185, used by advanceRK2/RK4/SSP3)."""
     ForwardEuler = 0
     SSPRK3 = 1
     RK4 = 2
     RK2 = 3
+    ModuleRK4 = 4
+    ModuleSSP3 = 5
 
 
 class ExplicitStepper:
@@ -603,14 +620,17 @@ def advanceRK2(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showProg
 
 def advanceRK4(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showProgress=False,
                use_graph=True):
-    """pressiodemoapps.advanceRK4 (pressiodemoapps/__init__.py:117-152), on the device."""
-    _advance(StepScheme.RK4, appObj, state, dt, Nsteps, startTime, observer, use_graph)
+    """pressiodemoapps.advanceRK4 (pressiodemoapps/__init__.py:117-152), on the
+    device, with the module's own stage algebra (k_i = dt v_i, y + (k1 + 2 k2
+    + 2 k3 + k4)/6) rounded as numpy rounds it."""
+    _advance(StepScheme.ModuleRK4, appObj, state, dt, Nsteps, startTime, observer, use_graph)
 
 
 def advanceSSP3(appObj, state, dt, Nsteps, startTime=0.0, observer=None, showProgress=False,
                 use_graph=True):
-    """pressiodemoapps.advanceSSP3 (pressiodemoapps/__init__.py:155-185), on the device."""
-    _advance(StepScheme.SSPRK3, appObj, state, dt, Nsteps, startTime, observer, use_graph)
+    """pressiodemoapps.advanceSSP3 (pressiodemoapps/__init__.py:155-185), on the
+    device, with the module's own stage algebra rounded as numpy rounds it."""
+    _advance(StepScheme.ModuleSSP3, appObj, state, dt, Nsteps, startTime, observer, use_graph)
 
 
 # ------------------------------------------------------- implicit stepping

@@ -69,6 +69,24 @@ TRAJ = [
 ]
 
 
+# the reference's Python tests (tests_py): mesh directory + gold files per
+# case; meshes are stored gzip-compressed, golds as float64 .npy
+TESTS_PY = [
+    ("1d_sod/weno3", "", ["gold.txt"]),
+    ("1d_sod/weno5", "", ["gold.txt"]),
+    ("1d_shu_osher/weno5", "", ["gold.txt"]),
+    ("2d_sedov_full/weno3", "", ["p_gold.txt"]),
+    ("2d_sedov_full/weno5", "", ["p_gold.txt"]),
+    ("2d_swe_slip_wall/weno3", "", ["h_gold.txt"]),
+    ("2d_swe_slip_wall/weno5", "", ["h_gold.txt"]),
+    ("2d_burgers_periodic", "mesh", ["gold.txt"]),
+    ("2d_advection_diffusion_reaction", "mesh", ["gold1.txt", "gold2.txt"]),
+    ("3d_sedov_symmetry/firstorder_nxnynz_equal", "", ["gold_state.txt"]),
+    ("3d_sedov_symmetry/weno3_nxnynz_equal", "", ["gold_state.txt"]),
+    ("ccu_mesh_s3_5x5", "", []),
+]
+
+
 def sha(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
@@ -101,6 +119,24 @@ def main(ref):
         np.save(dst, np.loadtxt(src))
         manifest.append({"file": os.path.relpath(dst, HERE), "source": rel, "sha256": sha(src),
                          "note": "np.loadtxt of the source text, saved as float64 .npy"})
+    import gzip
+    for case, meshsub, golds in TESTS_PY:
+        dst = os.path.join(OUT, "tests_py", case.replace("/", "__"))
+        os.makedirs(dst, exist_ok=True)
+        for f in ("info.dat", "connectivity.dat", "coordinates.dat"):
+            src = os.path.join(ref, "tests_py", case, meshsub, f)
+            out = os.path.join(dst, f + ".gz")
+            with open(src, "rb") as fi, gzip.GzipFile(out, "wb", mtime=0) as fo:
+                fo.write(fi.read())
+            manifest.append({"file": os.path.relpath(out, HERE),
+                             "source": f"tests_py/{case}/{meshsub + '/' if meshsub else ''}{f}",
+                             "sha256": sha(src), "note": "gzip of the source file"})
+        for g in golds:
+            src = os.path.join(ref, "tests_py", case, g)
+            out = os.path.join(dst, g.replace(".txt", ".npy"))
+            np.save(out, np.loadtxt(src))
+            manifest.append({"file": os.path.relpath(out, HERE), "source": f"tests_py/{case}/{g}",
+                             "sha256": sha(src), "note": "np.loadtxt of the source text, saved as float64 .npy"})
     with open(os.path.join(OUT, "MANIFEST.json"), "w") as f:
         json.dump(manifest, f, indent=1)
     print(f"{len(manifest)} fixtures written to {OUT}")

@@ -73,7 +73,7 @@ def test_batched_rk4_matches_single_instances_and_oracle(oracle, torch):
     import pyoracle
     for i, (mesh, prob, params) in enumerate(inst):
         Ui = prob.initialCondition()
-        pda.advanceRK4(prob, Ui, dt, nsteps)
+        pda.ExplicitStepper(prob, pda.StepScheme.RK4).advance(Ui, dt, nsteps)
         torch.cuda.synchronize()
         assert cases.rel_err(batch.instance(U, i).cpu().numpy(), Ui.cpu().numpy()) <= 1e-13
     # one instance against the oracle's own RK4 run
@@ -182,6 +182,6 @@ def test_batch_at_config_scale(oracle, torch):
     torch.cuda.synchronize()
     for i in range(len(sizes)):
         Ui = batch.instance(U, i).clone()
-        pda.advanceRK4(batch._probs[i], Ui, 0.01, 5)
+        pda.ExplicitStepper(batch._probs[i], pda.StepScheme.RK4).advance(Ui, 0.01, 5)
         torch.cuda.synchronize()
         assert torch.equal(batch.instance(Ub, i), Ui), i

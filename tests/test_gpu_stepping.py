@@ -85,7 +85,7 @@ def test_rk4_graph_matches_eager_and_oracle(case, oracle, torch):
     dt, nsteps = 1e-3, 60
     U0 = torch.from_numpy(cases.perturbed_state(prob, 2)).cuda()
     Ue, Ug = U0.clone(), U0.clone()
-    pda.advanceRK4(prob, Ue, dt, nsteps, use_graph=False)
+    pda.ExplicitStepper(prob, pda.StepScheme.RK4).advance(Ue, dt, nsteps, use_graph=False)
     # graph replayed in chunks with eager work in between
     st = pda.ExplicitStepper(prob, pda.StepScheme.RK4)
     V = prob.createRightHandSide()
@@ -130,20 +130,21 @@ def test_adr_rk4_matches_reference_gold(scheme, st, name, torch):
     mesh = pda.create_structured_mesh((18, 18), (0.0, 1.0, 0.0, 1.0), st)
     prob = pda.create_problem(mesh, pda.AdvectionDiffusionReaction2d.ProblemA, scheme)
     U = prob.initialCondition()
-    pda.advanceRK4(prob, U, 0.01, 500)
+    pda.ExplicitStepper(prob, pda.StepScheme.RK4).advance(U, 0.01, 500)  # pressio's RK4, as the C++ test
     gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref", "traj",
                                 f"eigen_2d_advdiffreac_probA_explicit__{name}__gold.npy"))
     assert np.allclose(U.cpu().numpy(), gold, rtol=1e-10, atol=1e-12)
 
 
-@pytest.mark.parametrize("scheme", ["ForwardEuler", "SSPRK3", "RK4", "RK2"])
+@pytest.mark.parametrize("scheme", ["ForwardEuler", "SSPRK3", "RK4", "RK2", "ModuleRK4", "ModuleSSP3"])
 def test_stepper_matches_stage_formulas(scheme, torch):
     """The device stepper on a problem evaluated by the generic row kernel
     (1D here, where SSPRK3 and RK4 stages are formed in the velocity
     launches) equals the stage formulas evaluated through rightHandSide and
     separate vector updates (pressio's forward Euler, SSPRK3 and RK4
-    stages, ode_explicit_stepper_without_mass_matrix.hpp:168-352; RK2 as
-    pressiodemoapps/__init__.py:90-114), 40 steps of Sod on 1000 cells."""
+    stages, ode_explicit_stepper_without_mass_matrix.hpp:168-352; RK2, RK4
+    and SSP3 as pressiodemoapps/__init__.py:90-185, whose algebra the
+    device reproduces bit for bit), 40 steps of Sod on 1000 cells."""
     print(f"[gpu-step] stepper vs stage formulas, {scheme}", flush=True)
     mesh = pda.create_structured_mesh((1000,), (-0.5, 0.5), 7)
     prob = pda.create_problem(mesh, pda.Euler1d.Sod, S.Weno5)
@@ -172,12 +173,29 @@ def test_stepper_matches_stage_formulas(scheme, torch):
             k3 = f(y + (dt / 2) * k2)
             k4 = f(y + dt * k3)
             y = y + (dt / 6) * k1 + (dt / 3) * k2 + (dt / 3) * k3 + (dt / 6) * k4
+        elif scheme == "RK2":
+            # pressiodemoapps/__init__.py:90-114, one rounding per operation
+            k1 = dt * f(y)
+            k2 = dt * f(y + k1)
+            y = y + k2 * 0.5 + k1 * 0.5
+        elif scheme == "ModuleRK4":
+            # pressiodemoapps/__init__.py:117-152
+            k1 = dt * f(y)
+            k2 = dt * f(y + 0.5 * k1)
+            k3 = dt * f(y + 0.5 * k2)
+            k4 = dt * f(y + k3)
+            y = y + (k1 + 2. * k2 + 2. * k3 + k4) * (1. / 6.)
         else:
-            k1 = f(y)
-            k2 = f(y + dt * k1)
-            y = y + (0.5 * dt) * k2 + (0.5 * dt) * k1
+            # pressiodemoapps/__init__.py:155-185
+            y1 = y + dt * f(y)
+            y2 = (3. / 4.) * y + (1. / 4.) * y1 + ((1. / 4.) * dt) * f(y1)
+            y = (1. / 3.) * (y + 2. * y2 + (2. * dt) * f(y2))
     torch.cuda.synchronize()
     assert torch.equal(U, Ug)  # graph replay == eager
     err = cases.rel_err(U.cpu().numpy(), y.cpu().numpy())
     print(f"[gpu-step] {scheme}: vs stage formulas {err:.2e}", flush=True)
-    assert err <= 1e-13
+    if scheme in ("RK2", "ModuleRK4", "ModuleSSP3"):
+        # the module's algebra, rounded as torch (and numpy) round it: bit for bit
+        assert torch.equal(U, y), err
+    else:
+        assert err <= 1e-13
