@@ -89,6 +89,12 @@ const int32_t *pda_mesh_graph(pda_mesh mesh);      /* n_sample x ncols */
 const double *pda_mesh_coords(pda_mesh mesh, int axis); /* n_stencil */
 /* counts of inner / near-boundary graph rows, impl/mesh_ccu.hpp:385-447 */
 int pda_mesh_row_classes(pda_mesh mesh, int64_t *n_inner, int64_t *n_near);
+/* graph rows of a row class, in row order, as the reference mesh lists them
+   (impl/mesh_ccu.hpp:385-447): which = 0 graphRowsOfCellsNearBd (a -1 within
+   the mesh stencil width), 1 graphRowsOfCellsAwayFromBd, 2
+   graphRowsOfCellsStrictlyOnBd (a missing first-order neighbour; 2D meshes
+   only, else PDA_E_UNSUPPORTED).  rows (nullable) receives *count ids. */
+int pda_mesh_boundary_rows(pda_mesh mesh, int which, int32_t *rows, int64_t *count);
 void pda_mesh_destroy(pda_mesh mesh);
 
 /* ------------------------------------------------------------- problem */

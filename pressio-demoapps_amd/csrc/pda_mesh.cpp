@@ -426,6 +426,36 @@ int pda_mesh_row_classes(pda_mesh m, int64_t *n_inner, int64_t *n_near) {
   return PDA_OK;
 }
 
+int pda_mesh_boundary_rows(pda_mesh m, int which, int32_t *rows, int64_t *count) {
+  if (!m || !count || which < 0 || which > 2) {
+    set_error("pda_mesh_boundary_rows: invalid argument");
+    return PDA_E_INVALID_ARG;
+  }
+  if (which == 2 && m->dim != 2) {
+    // impl/mesh_ccu.hpp:296-299 throws for meshes other than 2D
+    set_error("pda_mesh_boundary_rows: cells strictly next to the boundary are defined for 2D meshes only");
+    return PDA_E_UNSUPPORTED;
+  }
+  int64_t n = 0;
+  for (int64_t r = 0; r < m->n_sample; ++r) {
+    const bool near = mesh_near_bd(*m, r);
+    bool take = which == 0 ? near : !near;
+    if (which == 2) {
+      // a first-order neighbour is missing: near rows whose face itself is
+      // on the boundary, impl/mesh_ccu.hpp:276-305,440-445
+      take = false;
+      for (int a = 0; a < 2 && !take; ++a)
+        for (int side = 0; side < 2 && !take; ++side) take = m->g(r, col_of(2, a, side, 0)) == -1;
+    }
+    if (take) {
+      if (rows) rows[n] = int32_t(r);
+      ++n;
+    }
+  }
+  *count = n;
+  return PDA_OK;
+}
+
 void pda_mesh_destroy(pda_mesh m) { mesh_release(m); }
 
 }  // extern "C"

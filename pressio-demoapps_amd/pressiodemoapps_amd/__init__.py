@@ -69,6 +69,7 @@ def lib():
             "pda_mesh_graph": ([vp], P(ctypes.c_int32)),
             "pda_mesh_coords": ([vp, i32], P(dbl)),
             "pda_mesh_row_classes": ([vp, P(i64), P(i64)], i32),
+            "pda_mesh_boundary_rows": ([vp, i32, P(ctypes.c_int32), P(i64)], i32),
             "pda_mesh_destroy": ([vp], None),
             "pda_problem_create": ([vp, i32, i32, i32, i32, P(ctypes.c_char_p), P(dbl), i32, P(vp)], i32),
             "pda_problem_destroy": ([vp], None),
@@ -263,6 +264,25 @@ class CellCenteredUniformMesh:
         ni, nn = ctypes.c_int64(), ctypes.c_int64()
         _check(lib().pda_mesh_row_classes(self._h, ctypes.byref(ni), ctypes.byref(nn)))
         return ni.value, nn.value
+
+    def _rows(self, which):
+        n = ctypes.c_int64()
+        _check(lib().pda_mesh_boundary_rows(self._h, which, None, ctypes.byref(n)))
+        out = np.zeros(n.value, dtype=np.int32)
+        _check(lib().pda_mesh_boundary_rows(self._h, which, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                            ctypes.byref(n)))
+        return out
+
+    def graphRowsOfCellsNearBd(self):
+        """rows with a boundary within the mesh stencil width (impl/mesh_ccu.hpp:385-447)"""
+        return self._rows(0)
+
+    def graphRowsOfCellsAwayFromBd(self):
+        return self._rows(1)
+
+    def graphRowsOfCellsStrictlyOnBd(self):
+        """2D: rows with a missing first-order neighbour (impl/mesh_ccu.hpp:276-305)"""
+        return self._rows(2)
 
     def write(self, directory):
         os.makedirs(directory, exist_ok=True)

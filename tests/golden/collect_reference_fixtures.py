@@ -87,6 +87,23 @@ TESTS_PY = [
 ]
 
 
+# tests_mesh/mesh_boundary_cells_tagging_2d: the six meshes its main.cc
+# loads and the gold GID lists it holds (written as JSON)
+TAGGING = "tests_mesh/mesh_boundary_cells_tagging_2d"
+TAGGING_MESHES = ["fullmesh_s3", "fullmesh_s5", "fullmesh_s7", "samplemesh_s3", "samplemesh_s5", "samplemesh_s7"]
+
+
+def tagging_golds(main_cc):
+    """name -> list of GIDs, from the `const std::vector<int> name({...});`
+    gold declarations of the test's main.cc"""
+    import re
+    txt = open(main_cc).read()
+    out = {}
+    for name, body in re.findall(r"const std::vector<int>\s+(\w+)\s*\(\{([^}]*)\}\);", txt):
+        out[name] = [int(v) for v in body.replace("\n", " ").split(",") if v.strip()]
+    return out
+
+
 def sha(path):
     return hashlib.sha256(open(path, "rb").read()).hexdigest()
 
@@ -137,6 +154,20 @@ def main(ref):
             np.save(out, np.loadtxt(src))
             manifest.append({"file": os.path.relpath(out, HERE), "source": f"tests_py/{case}/{g}",
                              "sha256": sha(src), "note": "np.loadtxt of the source text, saved as float64 .npy"})
+    for n in TAGGING_MESHES:
+        dst = os.path.join(OUT, "mesh", "mesh_boundary_cells_tagging_2d", n)
+        os.makedirs(dst, exist_ok=True)
+        for f in ("info.dat", "connectivity.dat", "coordinates.dat"):
+            src = os.path.join(ref, TAGGING, n, f)
+            shutil.copyfile(src, os.path.join(dst, f))
+            manifest.append({"file": os.path.relpath(os.path.join(dst, f), HERE),
+                             "source": f"{TAGGING}/{n}/{f}", "sha256": sha(src)})
+    src = os.path.join(ref, TAGGING, "main.cc")
+    dst = os.path.join(OUT, "mesh", "mesh_boundary_cells_tagging_2d", "gold_gids.json")
+    with open(dst, "w") as f:
+        json.dump(tagging_golds(src), f, indent=1)
+    manifest.append({"file": os.path.relpath(dst, HERE), "source": f"{TAGGING}/main.cc", "sha256": sha(src),
+                     "note": "the gold GID lists declared in the source, as JSON"})
     with open(os.path.join(OUT, "MANIFEST.json"), "w") as f:
         json.dump(manifest, f, indent=1)
     print(f"{len(manifest)} fixtures written to {OUT}")

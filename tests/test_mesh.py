@@ -112,3 +112,39 @@ def test_load_errors_are_reported(tmp_path):
         pda.load_cellcentered_uniform_mesh(str(tmp_path / "missing"))
     with pytest.raises(pda.PdaError):
         pda.create_structured_mesh((4, 4), (0, 1, 0, 1), 4)
+
+
+TAGGING = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref", "mesh",
+                       "mesh_boundary_cells_tagging_2d")
+
+
+@pytest.mark.parametrize("kind", ["fullmesh", "samplemesh"])
+@pytest.mark.parametrize("s", [3, 5, 7])
+def test_boundary_cell_tagging_golds(kind, s):
+    """tests_mesh/mesh_boundary_cells_tagging_2d/main.cc: the cells near the
+    boundary (a -1 within the stencil width) and strictly next to it (a
+    missing first-order neighbour), as graph(row, 0) ids in row order, on the
+    reference's full and sample meshes of stencil 3, 5 and 7."""
+    import json
+    gold = json.load(open(os.path.join(TAGGING, "gold_gids.json")))
+    mesh = pda.load_cellcentered_uniform_mesh(os.path.join(TAGGING, f"{kind}_s{s}"))
+    G = mesh.graph()
+    near = G[mesh.graphRowsOfCellsNearBd(), 0].tolist()
+    strict = G[mesh.graphRowsOfCellsStrictlyOnBd(), 0].tolist()
+    if kind == "fullmesh":
+        want_near = gold[f"goldCellsGIDsNearBoundaries_fullS{s}"]
+        want_strict = gold["goldCellsGIDsStrictlyAtBoundaries_full"]
+    else:
+        want_near = gold[f"goldCellsGIDsNearBoundaries_sampleS{s}"]
+        want_strict = gold[f"goldCellsGIDsStrictlyAtBoundaries_sampleS{s}"]
+    assert near == want_near
+    assert strict == want_strict
+    away = mesh.graphRowsOfCellsAwayFromBd()
+    assert len(away) + len(near) == mesh.sampleMeshSize()
+    assert mesh.row_classes() == (len(away), len(near))
+
+
+def test_strictly_on_boundary_is_2d_only():
+    m = pda.create_structured_mesh((6, 5, 4), (0, 1, 0, 1, 0, 1), 3)
+    with pytest.raises(pda.PdaError):
+        m.graphRowsOfCellsStrictlyOnBd()
