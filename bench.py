@@ -4,8 +4,9 @@ Workload (the headline configuration): 2D Euler four-quadrant Riemann problem
 (Riemann IC2), 1024 x 1024 cells on [0,1]^2, gamma 1.4, stencil-7 mesh, FP64.
 One step = one seeded +-1% perturbation U*(1+delta) of the initial condition,
 evaluated as WENO5 velocity() (problem A) plus first-order exact-Rusanov
-jacobian() (problem B, 4x4 blocks, 5-point stencil), all inputs resident in
-HBM before the timed region.  Metric: residual+Jacobian cells/s.
+jacobian() (problem B, 4x4 blocks, 5-point stencil), the two calls issued on
+two streams, all inputs resident in HBM before the timed region.  Metric:
+residual+Jacobian cells/s.
 
 Prints one JSON line (the last line of stdout).
 """
@@ -298,15 +299,29 @@ def main():
     rps, rps_us = probA.autotune(states[0])
     log(f"[bench] velocity strip height {rps} ({rps_us:.1f} us per call while tuning)")
 
+    # One step issues problem A's rightHandSide on one stream and problem
+    # B's jacobian on another, as a caller may (the calls read the same
+    # state and write different outputs; each stream keeps its own calls in
+    # order): the velocity waves fill the compute units that the
+    # store-bound Jacobian leaves idle.
+    sV, sJ = torch.cuda.Stream(), torch.cuda.Stream()
+
     def step(i):
+        U = states[i % npert]
+        with torch.cuda.stream(sV):
+            probA.rightHandSide(U, 0.0, V)
+        with torch.cuda.stream(sJ):
+            probB.jacobian(U, 0.0, J)
+
+    def step_serial(i):
         U = states[i % npert]
         probA.rightHandSide(U, 0.0, V)
         probB.jacobian(U, 0.0, J)
 
     # autotune() times back-to-back calls on one state, which stays in cache;
-    # in these steps each velocity reads a state that the previous step's
-    # 671 MB Jacobian write has evicted, and a lower strip height hides that
-    # latency better.  So the height is also timed in this loop (untimed
+    # in the steps each velocity reads a state that the previous step's
+    # 671 MB Jacobian write has evicted, and shares the GPU with a Jacobian,
+    # so the velocity's strip height is also timed in this loop (untimed
     # set-up: two passes of 3 + 12 steps per candidate, best pass kept).
     def ctx_us(h):
         probA.set_velocity_strip_height(h)
@@ -321,7 +336,7 @@ def main():
             torch.cuda.synchronize()
             best = min(best, (time.perf_counter() - t) / 12 * 1e6)
         return best
-    cands = sorted({2, 3, 4, 5, 6, 8, rps})
+    cands = sorted({3, 4, 6, 8, 10, 12, 16, rps})
     ctx = {h: ctx_us(h) for h in cands}
     rps = min(ctx, key=ctx.get)
     probA.set_velocity_strip_height(rps)
@@ -334,29 +349,37 @@ def main():
     torch.cuda.synchronize()
 
     log(f"[bench] timing {args.steps} steps")
-    # events bracket every fifth jacobian() launch (the roofline kernel) on
-    # the stream it runs on, so that they perturb the timed steps little;
-    # the first sampled launch is the third, not the one that follows the
-    # idle gap of the synchronize; the velocity is timed in its own pass
-    s = torch.cuda.current_stream()
-    every = 5
-    ev = {i: (KernelEvent(), KernelEvent())
-          for i in range(min(2, args.steps - 1), args.steps, every)}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        U = states[(args.warmup + i) % npert]
-        probA.rightHandSide(U, 0.0, V)
-        e = ev.get(i)
-        if e:
-            e[0].record(s)
-        probB.jacobian(U, 0.0, J)
-        if e:
-            e[1].record(s)
+        step(args.warmup + i)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    t_j = sum(a.elapsed_time(b) for a, b in ev.values()) / len(ev) * 1e-3
-    # velocity() alone, each call between its own events (outside the timed steps)
+
+    # the same steps with both calls on one stream, for reference
+    for i in range(3):
+        step_serial(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step_serial(args.warmup + i)
+    torch.cuda.synchronize()
+    wall_serial = time.perf_counter() - t0
+
+    # the roofline kernel in its own pass: jacobian() calls alone, each
+    # between fence-free events on the stream it runs on, each on another
+    # perturbation (the first call, after the synchronize, is not sampled)
+    s = torch.cuda.current_stream()
+    nj = min(args.steps, 20)
+    evj = [(KernelEvent(), KernelEvent()) for _ in range(nj)]
+    probB.jacobian(states[0], 0.0, J)
+    for i in range(nj):
+        evj[i][0].record(s)
+        probB.jacobian(states[(i + 1) % npert], 0.0, J)
+        evj[i][1].record(s)
+    torch.cuda.synchronize()
+    t_j = sum(a.elapsed_time(b) for a, b in evj) / nj * 1e-3
+    # velocity() alone, likewise
     nv = min(args.steps, 20)
     evv = [(KernelEvent(), KernelEvent()) for _ in range(nv)]
     for i in range(nv):
@@ -373,7 +396,8 @@ def main():
     ach_j = bytes_j / t_j / 1e9
     value = cells / (wall / args.steps)
     log(f"[bench] velocity {t_v*1e6:.1f} us ({bytes_v/t_v/1e9:.0f} GB/s), jacobian {t_j*1e6:.1f} us "
-        f"({ach_j:.0f} GB/s), step {ms_per_step*1e3:.1f} us, unit bytes {(bytes_v+bytes_j)/1e6:.0f} MB "
+        f"({ach_j:.0f} GB/s), step {ms_per_step*1e3:.1f} us on two streams "
+        f"({wall_serial / args.steps * 1e6:.1f} us on one), unit bytes {(bytes_v+bytes_j)/1e6:.0f} MB "
         f"-> {(bytes_v+bytes_j)/(wall/args.steps)/1e9:.0f} GB/s")
 
     if args.copy_calib:
@@ -420,6 +444,9 @@ def main():
         "hbm_frac_per_step": (bytes_v + bytes_j) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
         "velocity_us": t_v * 1e6,
         "velocity_rows_per_strip": rps,
+        "streams": "velocity and jacobian on two streams",
+        "serial_ms_per_step": wall_serial / args.steps * 1e3,
+        "serial_value": cells / (wall_serial / args.steps),
         "cpu_baseline": cpu,
     }
     out.update(extra)

@@ -1,0 +1,68 @@
+"""Timing of jacobian() on the structured workloads outside the headline:
+WENO Jacobians (the staged kernel) and first-order ones, each on a seeded
++-1 % perturbation of the problem's initial condition, with the algorithmic
+bytes (8 nnz + 8 m N: values written once, state read once) per call.
+
+    python -m pressiodemoapps_amd.jacobians [--cases euler2d-weno5,burgers-weno5] [--reps 5]
+
+Cases:
+  euler2d-weno5   2D Euler Riemann IC2, 1024^2, WENO5 (13 4x4 blocks per row)
+  euler2d-weno3   2D Euler Riemann IC2, 1024^2, WENO3
+  burgers-weno5   2D viscous Burgers, periodic, 512^2, WENO5 (the implicit workload's Jacobian)
+  swe-weno3       shallow water, slip walls, 512^2, WENO3
+  euler3d-weno3   3D Euler Sedov, 128^3, WENO3
+"""
+import argparse
+import json
+import sys
+
+from . import (AdvectionDiffusion2d, Euler2d, Euler3d, InviscidFluxReconstruction as S, Swe2d, create_problem,
+               create_structured_mesh)
+
+CASES = {
+    "euler2d-weno5": lambda: (create_structured_mesh((1024, 1024), (0.0, 1.0, 0.0, 1.0), 7), Euler2d.Riemann,
+                              S.Weno5, 2, None),
+    "euler2d-weno3": lambda: (create_structured_mesh((1024, 1024), (0.0, 1.0, 0.0, 1.0), 5), Euler2d.Riemann,
+                              S.Weno3, 2, None),
+    "burgers-weno5": lambda: (create_structured_mesh((512, 512), (-1.0, 1.0, -1.0, 1.0), 7, True),
+                              AdvectionDiffusion2d.BurgersPeriodic, S.Weno5, 1, {"diffusion": 5e-3}),
+    "swe-weno3": lambda: (create_structured_mesh((512, 512), (-5.0, 5.0, -5.0, 5.0), 5), Swe2d.SlipWall, S.Weno3,
+                          1, None),
+    "euler3d-weno3": lambda: (create_structured_mesh((128, 128, 128), (0.0, 1.2, 0.0, 1.2, 0.0, 1.2), 5),
+                              Euler3d.SedovSymmetry, S.Weno3, 0, None),
+}
+
+
+def measure(name, reps, log):
+    import torch
+    from .sedov3d import _time
+    mesh, enum, scheme, ic, params = CASES[name]()
+    p = create_problem(mesh, enum, scheme, ic, params)
+    U0 = p.initialCondition()
+    g = torch.Generator(device="cuda").manual_seed(0)
+    U = U0 * (1.0 + 0.01 * (torch.rand(U0.numel(), dtype=torch.float64, device="cuda", generator=g) * 2 - 1))
+    J = p.createJacobian()
+    V = p.createRightHandSide()
+    tj = _time(lambda: p.jacobian(U, 0.0, J), reps)
+    tvj = _time(lambda: p.rightHandSideAndJacobian(U, 0.0, V, J), reps)
+    m, n_rows, nst = p.numDofPerCell(), p.totalDofSampleMesh(), p.totalDofStencilMesh()
+    bytes_j = 8 * J.nonZeros() + 8 * nst
+    r = {"case": name, "rows": n_rows // m, "jacobian_nnz": J.nonZeros(), "jacobian_us": tj * 1e6,
+         "jacobian_GBps": bytes_j / tj / 1e9, "rhs_and_jacobian_us": tvj * 1e6,
+         "rhs_and_jacobian_GBps": (bytes_j + 8 * n_rows) / tvj / 1e9,
+         "finite": bool(torch.isfinite(J.values).all().item())}
+    log(f"[jacobians] {json.dumps(r)}")
+    return r
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="pressiodemoapps_amd.jacobians")
+    ap.add_argument("--cases", default="euler2d-weno5,burgers-weno5,swe-weno3")
+    ap.add_argument("--reps", type=int, default=5)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    print(json.dumps([measure(c, a.reps, log) for c in a.cases.split(",")]))
+
+
+if __name__ == "__main__":
+    main()
