@@ -153,9 +153,11 @@ def extra_workloads():
             r["training"] = meta
             out["implicit_burgers"] = r
             log(f"[bench] implicit Burgers: speedup over the previous-step guess: learned "
-                f"{r['speedup_learned_vs_previous_step']:.2f}, FP64 predictor (linear f) "
-                f"{r['speedup_bdf2_predictor_vs_previous_step']:.2f}, (quadratic f) "
-                f"{r['speedup_bdf2_quadratic_predictor_vs_previous_step']:.2f}")
+                f"{r['speedup_learned_vs_previous_step']:.2f} (order {r['learned'].get('order_mean', 0):.2f} on "
+                f"average; {r['speedup_learned_vs_quadratic_predictor']:.3f} x the quadratic predictor, "
+                f"{r['speedup_learned_vs_best_fixed_order']:.3f} x the best fixed order, {r['best_fixed_order']}); "
+                f"FP64 predictors of order 1 / 2 / 3 / 4: "
+                + " / ".join(f"{r[f'speedup_predictor_order_{k}_vs_previous_step']:.2f}" for k in (1, 2, 3, 4)))
         else:
             out["implicit_burgers"] = {"error": "no trained weights at " + learned.DEFAULT_WEIGHTS}
     except Exception as e:  # reported, never fatal

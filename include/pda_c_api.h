@@ -300,44 +300,57 @@ void pda_implicit_destroy(pda_implicit ip);
 /* ------------------------------------------------- learned Newton guess */
 
 /* A 4-layer 3x3 convolutional network (32 channels, BF16 weights and
-   activations, FP32 accumulation on MFMA) predicting the next state of a
-   periodic nx x ny two-component field stepped with BDF2:
-     base  = (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1})   (FP64)
-     guess = base + net(y_n, dt f_n/s_f, dt (f_n - f_{n-1})/s_g,
-                        (nu - nu_mean)/nu_scale) * (2/3) dt (f_n - f_{n-1})
-   where f_n = f(y_n), f_{n-1} = f(y_{n-1}) are the problem's velocity at the
-   two previous states (d_fn, d_fnm1).
-   No counterpart in the reference: it feeds pda_implicit_step's d_guess,
-   and Newton on the exact Jacobian still decides the converged state.
-   States are device arrays of nx*ny*2 doubles in cell order j*nx + i.
-   nx*ny must be a multiple of 512. */
+   activations, FP32 accumulation on MFMA) that picks, per BDF2 step of a
+   periodic nx x ny two-component field, the extrapolation order t in [1, 4]
+   of an FP64 predictor of y_{n+1}: the BDF2 relation with f(y_{n+1})
+   extrapolated in time with order k from the velocities f_n, ..., f_{n-k},
+     P_k = (4/3) y_n - (1/3) y_{n-1} + (2/3) dt E_k,
+     E_1 = 2 f_n - f_{n-1},  E_2 = 3 f_n - 3 f_{n-1} + f_{n-2},
+     E_3 = 4 f_n - 6 f_{n-1} + 4 f_{n-2} - f_{n-3},
+     E_4 = 5 f_n - 10 f_{n-1} + 10 f_{n-2} - 5 f_{n-3} + f_{n-4},
+   guess = P(t), linear between P_k and P_{k+1} for k = floor(t).  The
+   means m_j over the grid of the network's PDA_GUESSNET_NORDERS output
+   channels estimate what the step costs from P(T_j), T_j = 1 + j/2 (j =
+   0..6), relative to the mean over the orders; t = sum_j p_j T_j with p =
+   softmax(-m / 0.25), a soft arg-min.  Its inputs are the time differences dt D^k f /
+   s_k (k = 1, 2, 3), |dt D^4 f| / s_4 and (nu - nu_mean) / nu_scale.  No
+   counterpart in the reference: it feeds pda_implicit_step's d_guess, and
+   Newton on the exact Jacobian still decides the converged state.  States
+   are device arrays of nx*ny*2 doubles in cell order j*nx + i; d_f is a
+   host array of five device pointers {f_n, f_{n-1}, f_{n-2}, f_{n-3},
+   f_{n-4}} (the problem's velocity at y_n, ..., y_{n-4}).  nx*ny must be a
+   multiple of 512 and at most 2^30. */
+#define PDA_GUESSNET_NORDERS 7
 typedef struct pda_guessnet_s *pda_guessnet;
 int pda_guessnet_create(int nx, int ny, pda_guessnet *out);
 void pda_guessnet_destroy(pda_guessnet net);
 int pda_guessnet_num_params(pda_guessnet net, int64_t *n);
 /* FP32 master parameters, per layer: W[tap][cout][cin] then bias[cout];
-   layers (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 2).  Setting them
+   layers (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 7).  Setting them
    resets the optimizer state. */
 int pda_guessnet_set_params(pda_guessnet net, const float *w, int64_t n);
 int pda_guessnet_get_params(pda_guessnet net, float *w, int64_t n);
-/* dt: the BDF2 time step; s_f, s_g, s_o: spreads of dt f_n, dt (f_n - f_{n-1})
-   and y_{n+1} - base; all positive */
-int pda_guessnet_set_scales(pda_guessnet net, double dt, double s_f, double s_g, double s_o,
-                            double nu_mean, double nu_scale);
+/* dt: the BDF2 time step; spreads[4]: of dt D^k f, k = 1..4; all positive */
+int pda_guessnet_set_scales(pda_guessnet net, double dt, const double *spreads, double nu_mean,
+                            double nu_scale);
 int pda_guessnet_predict(pda_guessnet net, const double *d_un, const double *d_unm1,
-                         const double *d_fn, const double *d_fnm1, double nu, double *d_guess,
-                         pda_stream stream);
-/* loss = mean(((net * gate - (y_{n+1} - base))/s_o)^2), gate =
-   (2/3) dt (f_n - f_{n-1}), and its gradient w.r.t. the parameters (host
-   arrays, nullable; out = the net's raw output, nx*ny*2 floats). */
-int pda_guessnet_gradients(pda_guessnet net, const double *d_un, const double *d_unm1,
-                           const double *d_fn, const double *d_fnm1, const double *d_unp1, double nu,
-                           float *grad, float *loss, float *out, pda_stream stream);
+                         const double *const *d_f, double nu, double *d_guess, pda_stream stream);
+/* order t and (nullable, PDA_GUESSNET_NORDERS values each) the weights p_j
+   and the cost estimates m_j of the last predict / gradients / train_step
+   call (synchronizes) */
+int pda_guessnet_policy(pda_guessnet net, double *order, double *probs, double *estimates);
+/* Training loss of one sample: sum_j (m_j - (costs[j] - mean costs))^2 / 7
+   for what the step cost from each P(T_j) (host, PDA_GUESSNET_NORDERS
+   finite values, e.g. Newton + Krylov iterations), and its gradient w.r.t.
+   the parameters (host arrays, nullable; out = the net's raw output,
+   nx*ny*PDA_GUESSNET_NORDERS floats). */
+int pda_guessnet_gradients(pda_guessnet net, const double *const *d_f, double nu,
+                           const double *costs, float *grad, float *loss, float *out,
+                           pda_stream stream);
 /* one Adam step (beta 0.9 / 0.999) on that loss; loss (nullable) is
    reported before the update */
-int pda_guessnet_train_step(pda_guessnet net, const double *d_un, const double *d_unm1,
-                            const double *d_fn, const double *d_fnm1, const double *d_unp1, double nu,
-                            float lr, float *loss, pda_stream stream);
+int pda_guessnet_train_step(pda_guessnet net, const double *const *d_f, double nu,
+                            const double *costs, float lr, float *loss, pda_stream stream);
 
 /* Block view of the Jacobian (m x m blocks).  For each block-row r and block
    slot s < nblk[r], d_bcol[r*maxb+s] is the column cell and the block values

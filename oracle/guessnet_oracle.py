@@ -3,12 +3,20 @@ tests/ use it to check the device kernels; the product never imports it).
 
 The network has no counterpart in the reference; this file states its
 definition directly: periodic 3x3 convolutions on nx*ny two-component
-fields, layers (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 2), SiLU
+fields, layers (cin, cout) = (8, 32), (32, 32), (32, 32), (32, 7), SiLU
 after the first three, flat FP32 parameters per layer W[tap][cout][cin] then
-bias[cout], tap = 3*(dy+1) + (dx+1) reading pixel (y+dy, x+dx).  The guess
-is base + output * gate with the FP64 BDF2 base
-(4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1}) and the gate
-(2/3) dt (f_n - f_{n-1}).
+bias[cout], tap = 3*(dy+1) + (dx+1) reading pixel (y+dy, x+dx).
+
+The guess is the FP64 predictor P(t) of y_{n+1} of extrapolation order t:
+the BDF2 relation with f(y_{n+1}) extrapolated in time with order k,
+P_k = (4/3) y_n - (1/3) y_{n-1} + (2/3) dt E_k(f_n, ..., f_{n-k}), linear in
+t between P_floor(t) and P_floor(t)+1.  The grid means m_j of the
+network's 7 output channels estimate the step's cost from P(T_j), T_j = 1
++ j/2, relative to their mean; t = sum_j p_j T_j with p = softmax(-m /
+TAU).  The loss of a sample with costs c is sum_j (m_j - (c_j - mean
+c))^2 / 7.  Inputs: dt D^k f / s_k for k = 1, 2, 3
+(D^k the k-th backward time difference), (nu - nu_mean) / nu_scale,
+|dt D^4 f| / s_4.
 
 `bf16=True` rounds to BF16 (round to nearest even) where the device stores
 BF16: features, weights, hidden activations and the backward's dZ; all
@@ -17,7 +25,9 @@ the finite-difference test pins.
 """
 import numpy as np
 
-LAYERS = [(8, 32), (32, 32), (32, 32), (32, 2)]
+LAYERS = [(8, 32), (32, 32), (32, 32), (32, 7)]
+ORDERS = tuple(1.0 + 0.5 * j for j in range(7))
+TAU = 0.25
 
 
 def bf16(x):
@@ -62,29 +72,76 @@ def _dsilu(z):
     return s * (1.0 + z * (1.0 - s))
 
 
-def features(un, fn, fnm1, nu, H, W, dt, s_f, s_g, nu_mean, nu_scale, rb=True):
-    """Channels: u, v, dt f_u/s_f, dt f_v/s_f, dt (f_u - f_u,n-1)/s_g, the
-    same for v, nu_hat, 0."""
+def diffs(f):
+    """Backward time differences D^1..D^4 of f = [f_n, ..., f_{n-4}]."""
+    f0, f1, f2, f3, f4 = f
+    return (f0 - f1, f0 - 2.0 * f1 + f2, f0 - 3.0 * f1 + 3.0 * f2 - f3,
+            f0 - 4.0 * f1 + 6.0 * f2 - 4.0 * f3 + f4)
+
+
+def features(f, nu, H, W, dt, spreads, nu_mean, nu_scale, rb=True):
+    """Channels: dt D^k f / s_k for k = 1, 2, 3 (u, v each), nu_hat,
+    |dt D^4 f| / s_4 (the norm over the two components)."""
+    d = diffs(f)
     X = np.zeros((H, W, 8))
-    X[..., 0:2] = un.reshape(H, W, 2)
-    X[..., 2:4] = fn.reshape(H, W, 2) * (dt / s_f)
-    X[..., 4:6] = (fn - fnm1).reshape(H, W, 2) * (dt / s_g)
+    for k in range(3):
+        X[..., 2 * k:2 * k + 2] = (d[k] * (dt / spreads[k])).reshape(H, W, 2)
     X[..., 6] = (nu - nu_mean) / nu_scale
-    return bf16(X) if rb else X
+    X[..., 7] = np.sqrt(((d[3] * (dt / spreads[3])).reshape(H, W, 2) ** 2).sum(axis=-1))
+    if rb:
+        # the device stores each feature as BF16 of its FP32 value
+        return bf16(X.astype(np.float32))
+    return X
 
 
-def base(un, unm1, fn, fnm1, dt):
-    """BDF2 with f(y_{n+1}) extrapolated linearly in time."""
-    return (4.0 / 3.0) * un - (1.0 / 3.0) * unm1 + ((2.0 / 3.0) * dt) * (2.0 * fn - fnm1)
+def predictor(un, unm1, f, dt, order):
+    """BDF2 with f(y_{n+1}) extrapolated with the given order (1..4) from
+    f = [f_n, f_{n-1}, ...]: the binomial extrapolation sum_j (-1)^j
+    C(order+1, j+1) f_{n-j}."""
+    from math import comb
+    E = sum(((-1) ** j) * comb(order + 1, j + 1) * f[j] for j in range(order + 1))
+    return (4.0 / 3.0) * un - (1.0 / 3.0) * unm1 + ((2.0 / 3.0) * dt) * E
 
 
-def gate(fn, fnm1, dt):
-    """The base's extrapolation increment, which scales the output."""
-    return ((2.0 / 3.0) * dt) * (fn - fnm1)
+def extrap_weights(t):
+    """Weights of f_n, ..., f_{n-4} in the order-t extrapolation (linear in t
+    between the binomial weights of floor(t) and floor(t) + 1)."""
+    from math import comb
+    E = [[((-1) ** j) * comb(k + 1, j + 1) if j <= k else 0 for j in range(5)] for k in range(1, 5)]
+    t = min(max(float(t), 1.0), 4.0)
+    k = 3 if t >= 4.0 else int(t)
+    w = t - k
+    return [(1.0 - w) * E[k - 1][j] + w * E[k][j] for j in range(5)]
+
+
+def predictor_t(un, unm1, f, dt, t):
+    """P(t) as the device forms it."""
+    e = extrap_weights(t)
+    E = e[0] * f[0] + e[1] * f[1] + e[2] * f[2] + e[3] * f[3] + e[4] * f[4]
+    return (4.0 / 3.0) * un - (1.0 / 3.0) * unm1 + ((2.0 / 3.0) * dt) * E
+
+
+def estimates(w, Y3, bf=True):
+    """Grid means m_j of the output channels from the last hidden layer's
+    output Y3 [H,W,32]: the output layer is linear and its padding periodic,
+    so m_j = bias_j + sum_i (sum_tap W[tap][j][i]) mean(Y3[..., i]), with the
+    BF16 weights when bf."""
+    Wl, bl = unpack(w)[-1]
+    Wr = bf16(Wl) if bf else Wl
+    return bl + Wr.sum(axis=0) @ np.asarray(Y3, dtype=np.float64).reshape(-1, Wl.shape[2]).mean(axis=0)
+
+
+def policy(m):
+    """(t, p): p = softmax(-m / TAU) of the cost estimates m, t = sum_j p_j
+    T_j."""
+    m = np.asarray(m, dtype=np.float64)
+    p = np.exp(-(m - m.min()) / TAU)
+    p /= p.sum()
+    return float(min(max(p @ np.asarray(ORDERS), 1.0), 4.0)), p
 
 
 def forward(w, X0, bf=True):
-    """Returns (output [H,W,2], cache) for the backward."""
+    """Returns (output [H,W,7], cache) for the backward."""
     P = unpack(w)
     X = X0
     cache = []
@@ -102,19 +159,25 @@ def forward(w, X0, bf=True):
     return z, cache
 
 
-def loss_and_grad(w, un, unm1, fn, fnm1, unp1, nu, H, W, dt, s_f, s_g, s_o, nu_mean, nu_scale, bf=True):
-    X0 = features(un, fn, fnm1, nu, H, W, dt, s_f, s_g, nu_mean, nu_scale, rb=bf)
+def loss_and_grad(w, f, nu, costs, H, W, dt, spreads, nu_mean, nu_scale, bf=True):
+    """loss = sum_j (m_j - (c_j - mean c))^2 / 7, the network's output and
+    the loss's gradient w.r.t. the parameters.  With m the output means
+    (estimates()),
+    d loss / d m_j = 2 (m_j - (c_j - mean c)) / 7 and d loss / d out[p][j]
+    = (d loss / d m_j) / (H W) at every pixel."""
+    X0 = features(f, nu, H, W, dt, spreads, nu_mean, nu_scale, rb=bf)
     out, cache = forward(w, X0, bf)
-    t = ((unp1 - base(un, unm1, fn, fnm1, dt)) / s_o).reshape(H, W, 2)
-    gs = (gate(fn, fnm1, dt) / s_o).reshape(H, W, 2)
-    d = out * gs - t
     if bf:
-        d = d.astype(np.float32).astype(np.float64)
+        out = out.astype(np.float32).astype(np.float64)
+    c = np.asarray(costs, dtype=np.float64)
+    m = estimates(w, cache[-1][0], bf)
+    r = m - (c - c.mean())
+    loss = float(r @ r / len(ORDERS))
+    gm = 2.0 * r / len(ORDERS)
     npix = H * W
-    loss = float(np.sum(d * d) / (npix * 2))
-    dz = 2.0 * d * gs / (npix * 2)
+    dz = np.broadcast_to(gm / npix, (H, W, len(ORDERS))).copy()
     if bf:
-        dz = bf16(dz)
+        dz = bf16(dz.astype(np.float32))
     grads = []
     for li in range(len(LAYERS) - 1, -1, -1):
         X, Wr, z = cache[li]

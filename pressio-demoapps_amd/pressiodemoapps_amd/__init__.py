@@ -98,12 +98,12 @@ def lib():
             "pda_guessnet_num_params": ([vp, P(i64)], i32),
             "pda_guessnet_set_params": ([vp, P(ctypes.c_float), i64], i32),
             "pda_guessnet_get_params": ([vp, P(ctypes.c_float), i64], i32),
-            "pda_guessnet_set_scales": ([vp, dbl, dbl, dbl, dbl, dbl, dbl], i32),
-            "pda_guessnet_predict": ([vp, vp, vp, vp, vp, dbl, vp, vp], i32),
-            "pda_guessnet_gradients": ([vp, vp, vp, vp, vp, vp, dbl, P(ctypes.c_float), P(ctypes.c_float),
+            "pda_guessnet_set_scales": ([vp, dbl, P(dbl), dbl, dbl], i32),
+            "pda_guessnet_predict": ([vp, vp, vp, P(vp), dbl, vp, vp], i32),
+            "pda_guessnet_policy": ([vp, P(dbl), P(dbl), P(dbl)], i32),
+            "pda_guessnet_gradients": ([vp, P(vp), dbl, P(dbl), P(ctypes.c_float), P(ctypes.c_float),
                                         P(ctypes.c_float), vp], i32),
-            "pda_guessnet_train_step": ([vp, vp, vp, vp, vp, vp, dbl, ctypes.c_float, P(ctypes.c_float),
-                                         vp], i32),
+            "pda_guessnet_train_step": ([vp, P(vp), dbl, P(dbl), ctypes.c_float, P(ctypes.c_float), vp], i32),
             "pda_batch_create": ([P(vp), i32, P(vp)], i32),
             "pda_batch_destroy": ([vp], None),
             "pda_batch_sizes": ([vp, P(i64), P(i64)], i32),
@@ -707,11 +707,19 @@ class ImplicitStepper:
 # ------------------------------------------------------ learned Newton guess
 class GuessNet:
     """Learned initial guess for BDF2 steps of a periodic nx x ny
-    two-component field (pda_guessnet_*): guess = base + net(y_n, dt f_n/s_f,
-    dt (f_n - f_{n-1})/s_g, nu_hat) * (2/3) dt (f_n - f_{n-1}) with the FP64
-    base (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1}); a 4-layer
-    3x3 conv net with BF16 weights and FP32 accumulation on MFMA, trained on
-    the device.  fn, fnm1 = the problem's velocity at y_n and y_{n-1}."""
+    two-component field (pda_guessnet_*): the network picks, per step, the
+    extrapolation order t in [1, 4] of the FP64 predictor P(t) (the BDF2
+    relation with f(y_{n+1}) extrapolated in time with order k from f_n,
+    ..., f_{n-k}, linear in t between integer orders).  The grid means m_j
+    of its output channels estimate the step's cost from P(T_j) at the
+    orders ORDERS = (1, 1.5, ..., 4), relative to their mean; t = sum_j p_j
+    T_j with p = softmax(-m / 0.25);
+    a 4-layer 3x3 conv net with BF16 weights and FP32 accumulation on MFMA,
+    trained on the device on sum_j (m_j - (c_j - mean c))^2 / 7 of samples
+    with measured costs c.  f = [f_n, f_{n-1}, f_{n-2},
+    f_{n-3}, f_{n-4}], the problem's velocity at y_n, ..., y_{n-4}."""
+
+    ORDERS = tuple(1.0 + 0.5 * j for j in range(7))
 
     def __init__(self, nx, ny):
         h = ctypes.c_void_p()
@@ -721,7 +729,7 @@ class GuessNet:
         n = ctypes.c_int64()
         _check(lib().pda_guessnet_num_params(self._h, ctypes.byref(n)))
         self.nparams = n.value
-        self.scales = (5e-3, 1e-2, 1e-3, 1e-6, 5.5e-3, 4.5e-3)
+        self.scales = (5e-3, (1e-3, 1e-4, 1e-5, 1e-6), 5.5e-3, 4.5e-3)
 
     def set_params(self, w):
         w = np.ascontiguousarray(w, dtype=np.float32)
@@ -734,40 +742,67 @@ class GuessNet:
                                              len(w)))
         return w
 
-    def set_scales(self, dt, s_f, s_g, s_o, nu_mean, nu_scale):
-        self.scales = tuple(float(v) for v in (dt, s_f, s_g, s_o, nu_mean, nu_scale))
-        _check(lib().pda_guessnet_set_scales(self._h, *self.scales))
+    def set_scales(self, dt, spreads, nu_mean, nu_scale):
+        """dt; spreads of dt D^k f for k = 1..4; nu_mean, nu_scale of the
+        diffusion input."""
+        spreads = tuple(float(v) for v in spreads)
+        if len(spreads) != 4:
+            raise PdaError("GuessNet.set_scales: four spreads (of dt D^k f, k = 1..4)")
+        self.scales = (float(dt), spreads, float(nu_mean), float(nu_scale))
+        sp = (ctypes.c_double * 4)(*spreads)
+        _check(lib().pda_guessnet_set_scales(self._h, float(dt), sp, float(nu_mean), float(nu_scale)))
 
-    def _check_fields(self, **fields):
+    def _fields(self, f, **fields):
         n = 2 * self.nx * self.ny
         for name, t in fields.items():
             _dev_tensor(t, n, name)
+        if len(f) != 5:
+            raise PdaError("GuessNet: f must hold f_n, f_{n-1}, f_{n-2}, f_{n-3}, f_{n-4}")
+        for k, t in enumerate(f):
+            _dev_tensor(t, n, f"f_(n-{k})")
+        return (ctypes.c_void_p * 5)(*[t.data_ptr() for t in f])
 
-    def predict(self, un, unm1, fn, fnm1, nu, out=None):
-        self._check_fields(un=un, unm1=unm1, fn=fn, fnm1=fnm1)
+    def _costs(self, costs):
+        c = np.ascontiguousarray(costs, dtype=np.float64)
+        if c.shape != (len(self.ORDERS),) or not np.all(np.isfinite(c)):
+            raise PdaError(f"GuessNet: costs must be {len(self.ORDERS)} finite values (one per order)")
+        return c, c.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def predict(self, un, unm1, f, nu, out=None):
+        fp = self._fields(f, un=un, unm1=unm1)
         if out is None:
             out = _torch().empty_like(un)
         _dev_tensor(out, 2 * self.nx * self.ny, "out")
-        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), float(nu),
-                                          _ptr(out), _stream()))
+        _check(lib().pda_guessnet_predict(self._h, _ptr(un), _ptr(unm1), fp, float(nu), _ptr(out), _stream()))
         return out
 
-    def gradients(self, un, unm1, fn, fnm1, unp1, nu):
-        self._check_fields(un=un, unm1=unm1, fn=fn, fnm1=fnm1, unp1=unp1)
+    def policy(self):
+        """(t, p, m) of the last predict / gradients / train_step: the order,
+        the weights over ORDERS and the cost estimates."""
+        t = ctypes.c_double()
+        p = (ctypes.c_double * len(self.ORDERS))()
+        m = (ctypes.c_double * len(self.ORDERS))()
+        _check(lib().pda_guessnet_policy(self._h, ctypes.byref(t), p, m))
+        return t.value, np.array(p[:]), np.array(m[:])
+
+    def gradients(self, f, nu, costs):
+        """(loss, gradient, raw output [nx*ny*7]) of one sample with the
+        given costs at ORDERS."""
+        fp = self._fields(f)
+        q, qp = self._costs(costs)
         g = np.zeros(self.nparams, dtype=np.float32)
-        o = np.zeros(self.nx * self.ny * 2, dtype=np.float32)
+        o = np.zeros(self.nx * self.ny * len(self.ORDERS), dtype=np.float32)
         loss = ctypes.c_float()
         F = ctypes.POINTER(ctypes.c_float)
-        _check(lib().pda_guessnet_gradients(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), _ptr(unp1),
-                                            float(nu), g.ctypes.data_as(F), ctypes.byref(loss),
+        _check(lib().pda_guessnet_gradients(self._h, fp, float(nu), qp, g.ctypes.data_as(F), ctypes.byref(loss),
                                             o.ctypes.data_as(F), _stream()))
         return loss.value, g, o
 
-    def train_step(self, un, unm1, fn, fnm1, unp1, nu, lr, want_loss=False):
-        self._check_fields(un=un, unm1=unm1, fn=fn, fnm1=fnm1, unp1=unp1)
+    def train_step(self, f, nu, costs, lr, want_loss=False):
+        fp = self._fields(f)
+        q, qp = self._costs(costs)
         loss = ctypes.c_float()
-        _check(lib().pda_guessnet_train_step(self._h, _ptr(un), _ptr(unm1), _ptr(fn), _ptr(fnm1), _ptr(unp1),
-                                             float(nu), float(lr),
+        _check(lib().pda_guessnet_train_step(self._h, fp, float(nu), qp, float(lr),
                                              ctypes.byref(loss) if want_loss else None, _stream()))
         return loss.value if want_loss else None
 
@@ -780,7 +815,7 @@ class GuessNet:
 
 def _guessnet_layers():
     """(cin, cout) of the guess network's layers (pda_guessnet_set_params)."""
-    return [(8, 32), (32, 32), (32, 32), (32, 2)]
+    return [(8, 32), (32, 32), (32, 32), (32, 7)]
 
 
 # ----------------------------------------------------------------- batching

@@ -1,9 +1,16 @@
 """Learned initial guesses for implicit Newton steps of periodic 2D viscous
 Burgers (WENO5, BDF2), trained on trajectories this package's own solver
-generates, with the device network of GuessNet: the guess is the FP64 BDF2
-predictor (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1}) plus the
-network's correction of its remainder, which the network gives as a
-multiple of the predictor's extrapolation step (2/3) dt (f_n - f_{n-1}).
+generates, with the device network of GuessNet.  The FP64 predictors of
+y_{n+1} are the BDF2 relation with f(y_{n+1}) extrapolated in time with
+order k from f_n, ..., f_{n-k} (linear in a fractional order t between
+integer orders); the network picks the order t in [1, 4] per step, trained
+on what the solver measured: for each training sample the Newton + Krylov
+iterations c_j the step took from each of the orders 1, 1.5, ..., 4, which
+the network learns to estimate (relative to their mean); it steps with the
+order of the smallest estimate (a soft arg-min).  (Which
+order pays changes with the flow: the highest on smooth fields, a low one
+once steep fronts form, where higher-order extrapolation in time of a front
+crossing a cell amplifies grid-scale error.)
 
 Workload: 512 x 512 cells on [-1, 1]^2, diffusion nu ~ U(1e-3, 1e-2), Gaussian
 pulse of amplitude U(0.5, 1.5) centred at U(-0.5, 0.5)^2 (spread 0.15), BDF2
@@ -51,29 +58,56 @@ def burgers_mesh(n):
     return create_structured_mesh((n, n), (-1.0, 1.0, -1.0, 1.0), 7, True)
 
 
+ORDERS = GuessNet.ORDERS  # 1, 1.5, ..., 4: the orders a training sample is costed at
+# binomial extrapolation weights of f_n, ..., f_{n-4} for integer orders 1..4
+EXTRAP = {1: (2.0, -1.0, 0.0, 0.0, 0.0), 2: (3.0, -3.0, 1.0, 0.0, 0.0), 3: (4.0, -6.0, 4.0, -1.0, 0.0),
+          4: (5.0, -10.0, 10.0, -5.0, 1.0)}
+NAMED = {"bdf2": 1.0, "bdf2q": 2.0, "bdf2c": 3.0, "bdf2q4": 4.0}
+
+
+def extrap_weights(t):
+    """weights of f_n, ..., f_{n-4} in the order-t extrapolation, linear in
+    t between floor(t) and floor(t) + 1 (t clamped to [1, 4])"""
+    t = min(max(float(t), 1.0), 4.0)
+    k = 3 if t >= 4.0 else int(t)
+    w = t - k
+    return tuple((1.0 - w) * a + w * b for a, b in zip(EXTRAP[k], EXTRAP[k + 1]))
+
+
+def predictor(order, yn, ynm1, f, out):
+    """FP64 guess: the BDF2 relation (4/3) y_n - (1/3) y_{n-1} + (2/3) dt
+    f(y_{n+1}) with f(y_{n+1}) extrapolated in time with the given
+    (possibly fractional) order from f = [f_n, f_{n-1}, ...]."""
+    out.copy_(yn).mul_(4.0 / 3.0).add_(ynm1, alpha=-1.0 / 3.0)
+    for c, fk in zip(extrap_weights(order), f):
+        if c != 0.0:
+            out.add_(fk, alpha=(2.0 / 3.0) * DT * c)
+    return out
+
+
 def bdf2_predictor(yn, ynm1, fn, fnm1, out):
-    """FP64 base guess: the BDF2 relation with f(y_{n+1}) extrapolated
-    linearly in time, (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1})."""
-    out.copy_(yn).mul_(4.0 / 3.0).add_(ynm1, alpha=-1.0 / 3.0)
-    out.add_(fn, alpha=(4.0 / 3.0) * DT).add_(fnm1, alpha=-(2.0 / 3.0) * DT)
-    return out
+    """the linear extrapolation, (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (2 f_n - f_{n-1})"""
+    return predictor(1, yn, ynm1, [fn, fnm1], out)
 
 
-def bdf2_quadratic_predictor(yn, ynm1, fn, fnm1, fnm2, out):
-    """FP64 guess with f(y_{n+1}) extrapolated quadratically in time,
-    (4/3) y_n - (1/3) y_{n-1} + (2/3) dt (3 f_n - 3 f_{n-1} + f_{n-2})."""
-    out.copy_(yn).mul_(4.0 / 3.0).add_(ynm1, alpha=-1.0 / 3.0)
-    out.add_(fn, alpha=2.0 * DT).add_(fnm1, alpha=-2.0 * DT).add_(fnm2, alpha=(2.0 / 3.0) * DT)
-    return out
+def _order_of(guess):
+    if isinstance(guess, str):
+        return NAMED[guess]
+    return float(guess)
 
 
-def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_step=None, lin_rtol=1e-12):
+def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_step=None, lin_rtol=1e-12,
+                   orders=None):
     """BDF2 trajectory from the IC.  guess: None (previous step, y_n),
-    "extrap" (2 y_n - y_{n-1}), "bdf2" (bdf2_predictor), "bdf2q"
-    (bdf2_quadratic_predictor; bdf2_predictor on the step where f_{n-2}
-    does not exist yet) or a GuessNet.  keep: set of step indices k whose
-    (y_{k-1}, y_k, y_{k+1}, f_k, f_{k-1}) are returned.  Returns (final
-    state, per-step stats, kept samples)."""
+    "extrap" (2 y_n - y_{n-1}), an extrapolation order in [1, 4] (or one of
+    the names "bdf2" / "bdf2q" / "bdf2c" / "bdf2q4" for 1 / 2 / 3 / 4; on
+    the first steps the highest order the history allows) or a GuessNet (the
+    cubic predictor until five velocities exist).  keep: set of step indices
+    whose training samples are returned: (f = [f_k, ..., f_{k-4}], costs),
+    costs[j] = Newton + Krylov iterations of the step solved from the
+    predictor of order ORDERS[j] (the trajectory itself goes on from the
+    cubic one).  orders: a list that receives the network's order per step.
+    Returns (final state, per-step stats, kept samples)."""
     import torch
     ip = ImplicitStepper(prob, ImplicitScheme.BDF2)
     ip.set_options(StopRule.Residual, tol, 30, lin_rtol, 0)
@@ -82,29 +116,38 @@ def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_ste
     stats, samples = [], []
     g = torch.empty_like(U)
     need_f = keep is not None or (guess is not None and not (isinstance(guess, str) and guess == "extrap"))
-    fbuf = [torch.empty_like(U) for _ in range(3)] if need_f else None
+    hist = []  # f_k, f_{k-1}, ... (at most 5)
     for k in range(nsteps):
         yn = U.clone()
-        fn = fnm1 = fnm2 = None
         if need_f:
-            fn, fnm1, fnm2 = fbuf[k % 3], fbuf[(k + 2) % 3], fbuf[(k + 1) % 3]
+            fn = hist.pop() if len(hist) == 5 else torch.empty_like(U)
             prob.rightHandSide(yn, k * DT, fn)
+            hist.insert(0, fn)
         gs = None
         if prev is not None and guess is not None:
             if isinstance(guess, str) and guess == "extrap":
                 torch.mul(yn, 2.0, out=g)
                 g.sub_(prev)
-            elif isinstance(guess, str) and guess == "bdf2q" and k >= 2:
-                bdf2_quadratic_predictor(yn, prev, fn, fnm1, fnm2, g)
-            elif isinstance(guess, str):
-                bdf2_predictor(yn, prev, fn, fnm1, g)
+            elif isinstance(guess, GuessNet):
+                if len(hist) == 5:
+                    guess.predict(yn, prev, hist, nu, out=g)
+                    if orders is not None:
+                        orders.append(guess.policy()[0])
+                else:
+                    predictor(min(3, len(hist) - 1), yn, prev, hist, g)
             else:
-                guess.predict(yn, prev, fn, fnm1, nu, out=g)
+                predictor(min(_order_of(guess), len(hist) - 1), yn, prev, hist, g)
             gs = g
+        if keep is not None and k in keep and prev is not None and len(hist) == 5:
+            costs = []
+            y = torch.empty_like(U)
+            for t in ORDERS:
+                y.copy_(yn)
+                s = ip.step(y, k * DT, DT, prev=prev, guess=predictor(t, yn, prev, hist, torch.empty_like(U)))
+                costs.append(s["newton_iters"] + s["linear_iters"])
+            samples.append(([h.clone() for h in hist], np.asarray(costs, dtype=np.float64)))
         st = ip.step(U, k * DT, DT, prev=prev, guess=gs)
         stats.append(st)
-        if keep is not None and k in keep and prev is not None:
-            samples.append((prev.clone(), yn.clone(), U.clone(), fn.clone(), fnm1.clone()))
         if on_step:
             on_step(k, st)
         prev = yn
@@ -112,24 +155,23 @@ def run_trajectory(prob, nsteps, guess=None, nu=None, keep=None, tol=TOL, on_ste
 
 
 def generate(n, n_traj, nsteps, keep_every, seed, log=print):
-    """Training data: every keep_every-th (y_{k-1}, y_k, y_{k+1}, f_k,
-    f_{k-1}) of n_traj trajectories; list of (nu, sample)."""
+    """Training data: every keep_every-th step of n_traj trajectories, as
+    (nu, f = [f_k, ..., f_{k-4}], costs at ORDERS)."""
     rng = np.random.default_rng(seed)
     mesh = burgers_mesh(n)
     data = []
-    # from step keep_every on: the first steps, where the pulse steepens
-    # into shocks, have remainders orders of magnitude above the rest and
-    # would set the output scale s_o (and the network's BF16 noise floor)
     keep = set(range(keep_every, nsteps, keep_every))
     t0 = time.perf_counter()
     for i in range(n_traj):
         params = draw_params(rng)
         prob = make_problem(mesh, params)
-        _, stats, tr = run_trajectory(prob, nsteps, keep=keep)
-        data += [(params["diffusion"], t) for t in tr]
+        # the cubic predictor only speeds the Newton solves up: every step
+        # converges to the same tolerance whatever the guess
+        _, stats, tr = run_trajectory(prob, nsteps, guess="bdf2c", keep=keep)
+        data += [(params["diffusion"], f, c) for f, c in tr]
+        best = np.bincount([int(np.argmin(c)) for _, c in tr], minlength=len(ORDERS))
         log(f"[learned] trajectory {i + 1}/{n_traj} nu={params['diffusion']:.4f}: {len(tr)} samples, "
-            f"{np.mean([s['newton_iters'] for s in stats]):.2f} Newton/step, "
-            f"{time.perf_counter() - t0:.0f} s")
+            f"cheapest order histogram {best.tolist()}, {time.perf_counter() - t0:.0f} s")
     return data
 
 
@@ -149,55 +191,79 @@ def init_params(net, seed):
 
 
 def fit(net, data, steps, lr, seed, log=print):
-    """Adam on random samples, cosine learning-rate decay."""
+    """Adam on random samples, cosine learning-rate decay; the loss of a
+    sample is the squared error of the network's cost estimates against the
+    sample's measured costs (both relative to their mean over the orders)."""
     rng = np.random.default_rng(seed)
     losses = []
     for s in range(steps):
-        nu, (ym1, y0, yp1, f0, fm1) = data[int(rng.integers(len(data)))]
+        nu, f, c = data[int(rng.integers(len(data)))]
         lr_s = lr * 0.5 * (1.0 + np.cos(np.pi * s / steps))
         want = (s % 100 == 0) or s == steps - 1
-        loss = net.train_step(y0, ym1, f0, fm1, yp1, nu, max(lr_s, 1e-6), want_loss=want)
+        loss = net.train_step(f, nu, c, max(lr_s, 1e-6), want_loss=want)
         if want:
             losses.append((s, loss))
             log(f"[learned] step {s}: loss {loss:.4e}")
     return losses
 
 
+def evaluate(net, data):
+    """Mean over samples of the excess iterations (over the sample's
+    cheapest order) at the network's order, read off the sample's costs
+    (linear between the costed orders), and the same for each fixed order."""
+    exc, fixed = [], np.zeros(len(ORDERS))
+    scratch = None
+    for nu, f, c in data:
+        # the policy depends on f and nu only; the guess formed from
+        # (f_n, f_n) as stand-ins for (y_n, y_{n-1}) is discarded
+        scratch = net.predict(f[0], f[0], f, nu, out=scratch)
+        t = net.policy()[0]
+        exc.append(float(np.interp(t, ORDERS, c) - c.min()))
+        fixed += c - c.min()
+    return float(np.mean(exc)), (fixed / len(data)).tolist()
+
+
 def scales_from(data):
-    """(s_f, s_g, s_o): spreads of dt f_n, of dt (f_n - f_{n-1}) and of the
-    remainder y_{n+1} - bdf2_predictor over the data."""
-    import torch
-    f, gg, o = [], [], []
-    for _, (ym1, y0, yp1, f0, fm1) in data:
-        b = bdf2_predictor(y0, ym1, f0, fm1, torch.empty_like(y0))
-        f.append((DT * f0).flatten()[::97].cpu().numpy())
-        gg.append((DT * (f0 - fm1)).flatten()[::97].cpu().numpy())
-        o.append((yp1 - b).flatten()[::97].cpu().numpy())
-    return float(np.std(np.concatenate(f))), float(np.std(np.concatenate(gg))), float(np.std(np.concatenate(o)))
+    """Spreads of dt D^k f for k = 1..4 over the data (the network's input
+    scales)."""
+    d = [[] for _ in range(4)]
+    coef = ((1, -1), (1, -2, 1), (1, -3, 3, -1), (1, -4, 6, -4, 1))
+    for _, f, _ in data:
+        for k in range(4):
+            dk = sum(c * fj for c, fj in zip(coef[k], f))
+            d[k].append((DT * dk).flatten()[::97].cpu().numpy())
+    return tuple(float(np.std(np.concatenate(x))) for x in d)
 
 
 def save(path, net, meta):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez(path, params=net.get_params(), scales=np.asarray(net.scales, dtype=np.float64),
+    dt, spreads, nu_mean, nu_scale = net.scales
+    np.savez(path, params=net.get_params(),
+             scales=np.asarray([dt, *spreads, nu_mean, nu_scale], dtype=np.float64),
              meta=np.asarray(json.dumps(meta)))
 
 
 def load(path, n):
     z = np.load(path, allow_pickle=False)
     net = GuessNet(n, n)
-    net.set_scales(*[float(v) for v in z["scales"]])
+    sc = [float(v) for v in z["scales"]]
+    net.set_scales(sc[0], sc[1:5], sc[5], sc[6])
     net.set_params(z["params"])
     return net, json.loads(str(z["meta"]))
 
 
+def _fixed_name(t):
+    return f"predictor_order_{t:g}"
+
+
 def compare(net, n, nsteps, seed, log=print, lin_rtol=1e-12):
     """Same held-out instance stepped with the previous-step guess, the
-    linear extrapolation, the FP64 BDF2 predictor (f(y_{n+1}) extrapolated
-    linearly, and quadratically) and the network on top of the linear
-    one; every run converges each step to ||R|| <= TOL with the same Newton
-    and linear-solver settings.  Returns a dict of steps/s, Newton and
-    linear iterations, and the final-state differences to the previous-step
-    (cold start) run."""
+    linear extrapolation 2 y_n - y_{n-1}, the FP64 predictor at each fixed
+    order 1, 1.5, ..., 4 and the learned guess (the network's order per
+    step); every run converges each step to ||R|| <= TOL with the same
+    Newton and linear-solver settings.  Returns a dict of steps/s, Newton
+    and linear iterations, and the final-state differences to the
+    previous-step (cold start) run."""
     import torch
     rng = np.random.default_rng(seed)
     params = draw_params(rng)
@@ -205,31 +271,46 @@ def compare(net, n, nsteps, seed, log=print, lin_rtol=1e-12):
     prob = make_problem(mesh, params)
     out = {"params": params, "steps": nsteps, "lin_rtol": lin_rtol}
     finals = {}
-    methods = (("previous_step", None), ("extrapolation", "extrap"), ("bdf2_predictor", "bdf2"),
-               ("bdf2_quadratic_predictor", "bdf2q"), ("learned", net))
+    methods = [("previous_step", None), ("extrapolation", "extrap")]
+    methods += [(_fixed_name(t), t) for t in ORDERS]
+    methods += [("learned", net)]
     for name, g in methods:
         # warm-up (library set-up, first-touch allocations)
-        run_trajectory(prob, 3, guess=g, nu=params["diffusion"], lin_rtol=lin_rtol)
+        run_trajectory(prob, 6, guess=g, nu=params["diffusion"], lin_rtol=lin_rtol)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         U, stats, _ = run_trajectory(prob, nsteps, guess=g, nu=params["diffusion"], lin_rtol=lin_rtol)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
+        orders = None
+        if name == "learned":
+            # the network's orders, from a second (untimed) run: reading
+            # them back each step would add a host synchronization
+            orders = []
+            run_trajectory(prob, nsteps, guess=g, nu=params["diffusion"], lin_rtol=lin_rtol, orders=orders)
         finals[name] = U.cpu().numpy()
         out[name] = {"steps_per_s": nsteps / el,
                      "newton_per_step": float(np.mean([s["newton_iters"] for s in stats])),
                      "linear_per_step": float(np.mean([s["linear_iters"] for s in stats])),
                      "median_initial_residual": float(np.median([s["initial_residual_norm"] for s in stats[2:]])),
                      "max_final_residual": float(max(s["residual_norm"] for s in stats))}
+        if orders:
+            o = np.asarray(orders)
+            out[name]["order_mean"] = float(o.mean())
+            out[name]["order_min"] = float(o.min())
+            out[name]["order_max"] = float(o.max())
         log(f"[learned] {name}: {out[name]}")
     ref = finals["previous_step"]
     for name, _ in methods[1:]:
         out[name]["final_state_max_abs_diff_vs_cold_start"] = float(np.abs(finals[name] - ref).max())
     sp = lambda a, b: out[a]["steps_per_s"] / out[b]["steps_per_s"]  # noqa: E731
+    best = max((_fixed_name(t) for t in ORDERS), key=lambda k: out[k]["steps_per_s"])
+    out["best_fixed_order"] = best
     out["speedup_learned_vs_previous_step"] = sp("learned", "previous_step")
-    out["speedup_learned_vs_bdf2_predictor"] = sp("learned", "bdf2_predictor")
-    out["speedup_bdf2_predictor_vs_previous_step"] = sp("bdf2_predictor", "previous_step")
-    out["speedup_bdf2_quadratic_predictor_vs_previous_step"] = sp("bdf2_quadratic_predictor", "previous_step")
+    out["speedup_learned_vs_quadratic_predictor"] = sp("learned", _fixed_name(2.0))
+    out["speedup_learned_vs_best_fixed_order"] = sp("learned", best)
+    for t in ORDERS:
+        out[f"speedup_{_fixed_name(t)}_vs_previous_step"] = sp(_fixed_name(t), "previous_step")
     return out
 
 
@@ -238,9 +319,9 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="cmd", required=True)
     t = sub.add_parser("train")
     t.add_argument("--n", type=int, default=512)
-    t.add_argument("--trajectories", type=int, default=64)
+    t.add_argument("--trajectories", type=int, default=48)
     t.add_argument("--steps", type=int, default=400)
-    t.add_argument("--keep-every", type=int, default=25)
+    t.add_argument("--keep-every", type=int, default=10)
     t.add_argument("--train-steps", type=int, default=3000)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=2024)
@@ -254,15 +335,18 @@ def main(argv=None):
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     if a.cmd == "train":
         data = generate(a.n, a.trajectories, a.steps, a.keep_every, a.seed, log)
-        s_f, s_g, s_o = scales_from(data)
+        spreads = scales_from(data)
         net = GuessNet(a.n, a.n)
-        net.set_scales(DT, s_f, s_g, s_o, 5.5e-3, 4.5e-3)
+        net.set_scales(DT, spreads, 5.5e-3, 4.5e-3)
         init_params(net, a.seed + 1)
         losses = fit(net, data, a.train_steps, a.lr, a.seed + 2, log)
         meta = {"n": a.n, "trajectories": a.trajectories, "steps": a.steps, "keep_every": a.keep_every,
                 "samples": len(data), "train_steps": a.train_steps, "lr": a.lr, "seed": a.seed,
-                "s_f": s_f, "s_g": s_g, "s_o": s_o, "final_loss": losses[-1][1],
+                "spreads": list(spreads), "final_loss": losses[-1][1],
                 "mean_loss_last_1000": float(np.mean([l for st, l in losses if st >= a.train_steps - 1000]))}
+        exc, fixed = evaluate(net, data)
+        meta["training_excess_iterations_learned"] = exc
+        meta["training_excess_iterations_fixed_orders"] = fixed
         save(a.out, net, meta)
         print(json.dumps(meta))
     else:
