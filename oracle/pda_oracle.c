@@ -962,6 +962,46 @@ static void ghosts_adr(ctx_t *c, const double *U)
   }
 }
 
+/* Burgers 2D outflow, impl/advection_diffusion_2d_ghost_filler_outflow.hpp:
+   homogeneous Dirichlet (every ghost 0) at the left and back sides, the
+   mirror images about the boundary (homogeneous Neumann) at the right and
+   front sides, with the layer rules of the 2D Euler Neumann filler */
+static void ghosts_burgers_outflow(ctx_t *c, const double *U)
+{
+  const orc_mesh *M = c->mesh;
+  for (int64_t r = 0; r < M->nrows; ++r) {
+    const int64_t g = c->ghostRow[r];
+    if (g < 0) continue;
+    const int32_t self = G(M, r, 0);
+    for (int a = 0; a < 2; ++a)
+      for (int side = 0; side < 2; ++side) {
+        const double v = side == 0 ? 0. : 1.;
+        const double sgn[MAXM] = {v, v, v, v, v};
+        const int opp = 1 - side;
+        const int32_t n0 = G(M, r, col_of(2, a, side, 0));
+        const int32_t o0 = G(M, r, col_of(2, a, opp, 0));
+        double *dst = GH(c, a, side, g);
+        if (n0 == -1) copy_cell(dst, U, self, 2, sgn);
+        if (c->w >= 2) {
+          const int32_t n1 = G(M, r, col_of(2, a, side, 1));
+          if (n1 == -1) copy_cell(dst + 2, U, (n0 == -1) ? o0 : n0, 2, sgn);
+        }
+        if (c->w == 3) {
+          const int32_t n1 = G(M, r, col_of(2, a, side, 1));
+          const int32_t n2 = G(M, r, col_of(2, a, side, 2));
+          const int32_t o1 = G(M, r, col_of(2, a, opp, 1));
+          if (n2 == -1) {
+            int32_t src = self;
+            if (n1 != -1 && n0 != -1) src = n1;
+            if (n1 == -1 && n0 != -1) src = self;
+            if (n1 == -1 && n0 == -1) src = o1;
+            copy_cell(dst + 4, U, src, 2, sgn);
+          }
+        }
+      }
+  }
+}
+
 static int fill_ghosts(ctx_t *c, const double *U)
 {
   const orc_problem *pb = c->pb;
@@ -981,7 +1021,8 @@ static int fill_ghosts(ctx_t *c, const double *U)
     ghosts_mirror2d(c, U, 1);
     return 0;
   case ORC_BURGERS2D:
-    if (pb->prob != 0) return -3; /* only the periodic problem is restated */
+    if (pb->prob == 1) ghosts_burgers_outflow(c, U);
+    else if (pb->prob != 0) return -3;
     return 0;
   case ORC_ADR2D:
     ghosts_adr(c, U);
@@ -994,7 +1035,9 @@ static int fill_ghosts(ctx_t *c, const double *U)
    Euler 1D impl/euler_1d_prob_class.hpp:608-610; Euler 2D
    impl/euler_2d_prob_class.hpp:1115-1224; Euler 3D
    impl/euler_3d_prob_class.hpp:1017-1044; SWE impl/swe_2d_prob_class.hpp:839-854;
-   ADR impl/advection_diffusion_reaction_2d_prob_class.hpp (Dirichlet: -1). */
+   ADR impl/advection_diffusion_reaction_2d_prob_class.hpp (Dirichlet: -1);
+   Burgers outflow impl/advection_diffusion_2d_prob_class.hpp:1131-1156
+   (0 at the left and back boundaries, 1 at the right and front ones). */
 static void bc_factors(ctx_t *c, int64_t r, int a, double *f)
 {
   for (int k = 0; k < MAXM; ++k) f[k] = 1.;
@@ -1005,6 +1048,9 @@ static void bc_factors(ctx_t *c, int64_t r, int a, double *f)
     if (has_bd(c->mesh, r, a, 0)) f[1 + a] = -1.;
     break;
   case ORC_ADR2D: f[0] = -1.; break;
+  case ORC_BURGERS2D:
+    if (has_bd(c->mesh, r, a, 0)) f[0] = f[1] = 0.;
+    break;
   default: break;
   }
 }
@@ -1203,6 +1249,23 @@ static void sources(ctx_t *c, int64_t r, const double *U, double t, double *Vr, 
         jadd(c, row + k, uB + k, dY);
       }
     }
+  } else if (pb->family == ORC_BURGERS2D && withJac) {
+    /* diffusion Jacobian of a near-boundary cell (its velocity was added
+       after each axis, burgers_diffusion_near): the diagonal, then per
+       first neighbour (left, front, right, back) its weight, or minus the
+       weight on the diagonal where the neighbour is missing,
+       impl/advection_diffusion_2d_prob_class.hpp:752-790 */
+    const double two = 2;
+    const double dX = pb->p[0] * (c->hInv[0] * c->hInv[0]), dY = pb->p[0] * (c->hInv[1] * c->hInv[1]);
+    const int64_t row = r * m, ui = (int64_t)G(M, r, 0) * m;
+    const int32_t nb[4] = {G(M, r, 1), G(M, r, 2), G(M, r, 3), G(M, r, 4)};
+    const double wt[4] = {dX, dY, dX, dY};
+    for (int k = 0; k < 2; ++k) jadd(c, row + k, ui + k, -two * dX - two * dY);
+    for (int q = 0; q < 4; ++q)
+      for (int k = 0; k < 2; ++k) {
+        if (nb[q] > -1) jadd(c, row + k, (int64_t)nb[q] * m + k, wt[q]);
+        else jadd(c, row + k, ui + k, -wt[q]);
+      }
   } else if (pb->family == ORC_ADR2D) {
     /* ADR problem A, impl/advection_diffusion_reaction_2d_prob_class.hpp:
        inner rows add both axes' diffusion here (:1044-1049 with
@@ -1244,6 +1307,19 @@ static void sources(ctx_t *c, int64_t r, const double *U, double t, double *Vr, 
       }
     }
   }
+}
+
+/* Burgers diffusion of a near-boundary row along axis a from its stencil,
+   a missing neighbour read from the ghost layer,
+   impl/advection_diffusion_2d_prob_class.hpp:721-729 and :892-894 */
+static void burgers_diffusion_near(ctx_t *c, int64_t r, int a, const double *U, double *Vr)
+{
+  double st[3 * 2];
+  gather(c, r, a, 1, U, st);
+  const double two = 2;
+  const double d = c->pb->p[0] * (c->hInv[a] * c->hInv[a]);
+  Vr[0] += d * (st[4] - two * st[2] + st[0]);
+  Vr[1] += d * (st[5] - two * st[3] + st[1]);
 }
 
 /* ADR diffusion of a near-boundary row along axis a from its stencil, the
@@ -1299,6 +1375,7 @@ static void eval_row(ctx_t *c, int64_t r, const double *U, double t, double *V, 
       else jac_interior(c, r, a, &f, JLn, JLp, JRn, JRp);
     }
     if (pb->family == ORC_ADR2D && near) adr_diffusion_near(c, r, a, U, Vr);
+    if (pb->family == ORC_BURGERS2D && near) burgers_diffusion_near(c, r, a, U, Vr);
   }
   sources(c, r, U, t, Vr, withJac);
   for (int k = 0; k < m; ++k) V[r * m + k] = Vr[k];

@@ -124,14 +124,33 @@ template <> struct Sources<PhSwe> {
 };
 
 // Burgers viscous term on inner cells,
-// impl/advection_diffusion_2d_prob_class.hpp:1167-1204
+// impl/advection_diffusion_2d_prob_class.hpp:1167-1204; on near-boundary
+// cells (the outflow problem) its velocity is added after each axis by
+// AxisTerms<PhBurgers>, and its Jacobian here: the diagonal, then for each
+// first neighbour (left, front, right, back) minus its weight on the
+// diagonal where it is missing (a present one takes offdiag() in its
+// block), impl/advection_diffusion_2d_prob_class.hpp:752-790
 template <> struct Sources<PhBurgers> {
   __device__ static void apply(const View &P, int64_t r, bool near, const double *U, int32_t self,
                                double *v, double *J) {
-    if (near) return;
     const int32_t *gr = P.graph + r * P.ncols;
     const double dX = P.p[0] * (P.hinv[0] * P.hinv[0]);
     const double dY = P.p[0] * (P.hinv[1] * P.hinv[1]);
+    if (near) {
+      if (J) {
+#pragma clang fp contract(off)
+        const double w[4] = {dX, dY, dX, dY};
+        J[0] += -2. * dX - 2. * dY;
+        J[3] += -2. * dX - 2. * dY;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (gr[1 + q] == -1) {
+            J[0] += -w[q];
+            J[3] += -w[q];
+          }
+      }
+      return;
+    }
     const int64_t ui = 2 * int64_t(self), uL = 2 * int64_t(gr[1]), uF = 2 * int64_t(gr[2]);
     const int64_t uR = 2 * int64_t(gr[3]), uB = 2 * int64_t(gr[4]);
     for (int k = 0; k < 2; ++k) {
@@ -143,9 +162,10 @@ template <> struct Sources<PhBurgers> {
       J[3] += -2. * dX - 2. * dY;
     }
   }
-  // the diffusion adds to the diagonal of the four first-neighbour blocks
-  __device__ static double offdiag(const View &P, bool near, int a, int k, int j) {
-    if (near || k != j) return 0.;
+  // the diffusion adds to the diagonal of the first-neighbour blocks
+  // (all four on inner rows, the present ones on near-boundary rows)
+  __device__ static double offdiag(const View &P, bool, int a, int k, int j) {
+    if (k != j) return 0.;
     return P.p[0] * (P.hinv[a] * P.hinv[a]);
   }
 };
@@ -213,6 +233,29 @@ template <> struct AxisTerms<PhAdr> {
     const double u = U[gr[0]];
     const double ul = l >= 0 ? U[l] : -u, ur = r >= 0 ? U[r] : -u;
     acc[0] += (P.p[2] * (P.hinv[A] * P.hinv[A])) * (ur - 2. * u + ul);
+  }
+};
+
+// Burgers near-boundary diffusion along axis A from the cell's first
+// neighbours, a missing one read as its ghost (0 at the outflow problem's
+// left and back sides, the cell itself at the right and front ones: the
+// side's sign times the cell), impl/advection_diffusion_2d_prob_class.hpp:
+// 721-729 and :892-894
+template <> struct AxisTerms<PhBurgers> {
+  static constexpr bool near_offdiag = true;
+  template <int A>
+  __device__ static void velocity(const View &P, const int32_t *gr, const double *U, double *acc) {
+#pragma clang fp contract(off)
+    const int32_t l = gr[A == 0 ? 1 : 4], r = gr[A == 0 ? 3 : 2];
+    const int64_t s = 2 * int64_t(gr[0]);
+    const double d = P.p[0] * (P.hinv[A] * P.hinv[A]);
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const double u = U[s + k];
+      const double ul = l >= 0 ? U[2 * int64_t(l) + k] : P.gsign[A][0][k] * u;
+      const double ur = r >= 0 ? U[2 * int64_t(r) + k] : P.gsign[A][1][k] * u;
+      acc[k] += d * (ur - 2. * u + ul);
+    }
   }
 };
 

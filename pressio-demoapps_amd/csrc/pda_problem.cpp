@@ -105,6 +105,7 @@ int ghost_rule_of(int family, int prob, bool *supported) {
       break;
     case PDA_BURGERS2D:
       if (prob == 0) return GHOST_NONE;             // BurgersPeriodic
+      if (prob == 1) return GHOST_MIRROR;           // BurgersOutflow (0 at the minus sides, gsign)
       break;
     case PDA_ADR2D:
       if (prob == 0) return GHOST_OPPOSITE;         // ProblemA, homogeneous Dirichlet
@@ -268,12 +269,16 @@ int initial_condition_host(const pda_problem_s &p, std::vector<double> &U) {
 // Euler 1D impl/euler_1d_prob_class.hpp:608-610, Euler 2D
 // impl/euler_2d_prob_class.hpp:1115-1140, Euler 3D
 // impl/euler_3d_prob_class.hpp:1017-1044, SWE impl/swe_2d_prob_class.hpp:839-854,
-// ADR impl/advection_diffusion_reaction_2d_prob_class.hpp:640-641 and :809-812
+// ADR impl/advection_diffusion_reaction_2d_prob_class.hpp:640-641 and :809-812,
+// Burgers outflow impl/advection_diffusion_2d_prob_class.hpp:1131-1156
 void bc_factors(const pda_problem_s &p, int64_t r, int a, double *f) {
   for (int k = 0; k < 5; ++k) f[k] = 1.;
   if (p.family == PDA_ADR2D) f[0] = -1.;
   if (p.family == PDA_SWE2D) f[a == 0 ? 1 : 2] = -1.;
   if (p.family == PDA_EULER3D && p.prob == 1 && mesh_has_bd(*p.mesh, r, a, 0)) f[1 + a] = -1.;
+  // Burgers outflow: homogeneous Dirichlet at the left and back boundaries,
+  // homogeneous Neumann at the right and front ones
+  if (p.family == PDA_BURGERS2D && p.prob == 1 && mesh_has_bd(*p.mesh, r, a, 0)) f[0] = f[1] = 0.;
 }
 
 int build(pda_problem_s &p) {
@@ -418,6 +423,12 @@ int build(pda_problem_s &p) {
   if (p.family == PDA_ADR2D)
     for (int a = 0; a < 3; ++a)
       for (int s = 0; s < 2; ++s) v.gsign[a][s][0] = -1.;
+  // Burgers outflow: every ghost of the left and back sides is 0, those of
+  // the right and front sides mirror the cells inside
+  // (impl/advection_diffusion_2d_ghost_filler_outflow.hpp:105-231)
+  if (p.family == PDA_BURGERS2D && p.prob == 1)
+    for (int a = 0; a < 2; ++a)
+      for (int k = 0; k < 2; ++k) v.gsign[a][0][k] = 0.;
   return PDA_OK;
 }
 

@@ -18,6 +18,10 @@ for s in (S.FirstOrder, S.Weno3, S.Weno5):
          {"gravity": 9.8, "coriolis": -3.0}),
         (f"burgers-periodic-{s.name}", pda.AdvectionDiffusion2d.BurgersPeriodic, s, (18, 16),
          (-1, 1, -1, 1), 7, True, 1, {"diffusion": 1e-4}),
+        # homogeneous Dirichlet at the left and back sides, Neumann at the
+        # right and front ones; a pulse near the right/front corner
+        (f"burgers-outflow-{s.name}", pda.AdvectionDiffusion2d.BurgersOutflow, s, (18, 16),
+         (-1, 1, -1, 1), 7, (), 1, {"diffusion": 5e-3, "pulseX": 0.6, "pulseY": 0.5}),
     ]
 for s in (S.FirstOrder, S.Weno3):
     CASES += [
@@ -56,6 +60,10 @@ BIG_CASES = [
      True, 1, {"diffusion": 5e-3}),
     ("burgers-big-FirstOrder", pda.AdvectionDiffusion2d.BurgersPeriodic, S.FirstOrder, (130, 66), (-1, 1, -1, 1), 3,
      True, 1, {"diffusion": 5e-3}),
+    ("burgers-outflow-big-Weno5", pda.AdvectionDiffusion2d.BurgersOutflow, S.Weno5, (130, 66), (-1, 1, -1, 1), 7,
+     (), 1, {"diffusion": 5e-3, "pulseX": 0.7, "pulseY": 0.6}),
+    ("burgers-outflow-big-FirstOrder", pda.AdvectionDiffusion2d.BurgersOutflow, S.FirstOrder, (97, 70),
+     (-1, 1, -1, 1), 3, (), 1, {"diffusion": 5e-3, "pulseX": 0.7, "pulseY": 0.6}),
     # box lines of 38 cells: the 3D first-order Jacobian copies runs of 16
     # consecutive rows as one range
     ("euler3d-sedov-big-FirstOrder", pda.Euler3d.SedovSymmetry, S.FirstOrder, (40, 7, 6), (0, 1.2, 0, 1.2, 0, 1.2),

@@ -66,6 +66,9 @@ TRAJ = [
     "tests_cpp/eigen_2d_advdiffreac_probA_implicit/firstorder/gold.txt",
     "tests_cpp/eigen_2d_advdiffreac_probA_implicit/weno3/gold.txt",
     "tests_cpp/eigen_2d_advdiffreac_probA_implicit/weno5/gold.txt",
+    # Burgers outflow: Crank-Nicolson on 20x20 (the reference commits no WENO5 gold)
+    "tests_cpp/eigen_2d_burgers_outflow_implicit/firstorder/gold.txt",
+    "tests_cpp/eigen_2d_burgers_outflow_implicit/weno3/gold.txt",
 ]
 
 

@@ -112,6 +112,24 @@ def test_adr_crank_nicolson_matches_reference_gold(scheme, st, name, torch):
     assert np.allclose(Uh, gold, rtol=1e-10, atol=1e-12)
 
 
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3")])
+def test_burgers_outflow_crank_nicolson_matches_reference_gold(scheme, st, name, torch):
+    # tests_cpp/eigen_2d_burgers_outflow_implicit: 20x20 on [-1,1]^2, default
+    # pulse and diffusion, CN dt 0.01 x 200, Newton tolerance 1e-5
+    print(f"[gpu-implicit] burgers outflow cn {name}", flush=True)
+    mesh = pda.create_structured_mesh((20, 20), (-1.0, 1.0, -1.0, 1.0), st)
+    prob = pda.create_problem(mesh, pda.AdvectionDiffusion2d.BurgersOutflow, scheme, 1)
+    ip = pda.ImplicitStepper(prob, pda.ImplicitScheme.CrankNicolson)
+    ip.set_options(pda.StopRule.Correction, 1e-5)
+    U = prob.initialCondition()
+    for k in range(200):
+        ip.step(U, k * 0.01, 0.01)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_burgers_outflow_implicit__{name}__gold.npy"))
+    Uh = U.cpu().numpy()
+    print(f"[gpu-implicit] burgers outflow {name}: max |U-gold| {np.abs(Uh - gold).max():.2e}", flush=True)
+    assert np.allclose(Uh, gold, rtol=1e-10, atol=1e-12)
+
+
 @pytest.mark.parametrize("stop", ["residual", "correction"])
 def test_nonfinite_state_leaves_yn(stop, torch):
     """A NaN in y_n makes the residual non-finite: the step fails with

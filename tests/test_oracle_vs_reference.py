@@ -70,7 +70,7 @@ def test_product_setup_matches_reference(case, tmp_path):
 SAMPLE_CASES = [c for c in REF_CASES if c[0] in (
     "euler2d-riemann-FirstOrder", "euler2d-riemann-Weno5", "swe-slipwall-Weno3",
     "burgers-periodic-Weno5", "euler1d-sod-Weno5", "euler3d-sedov-Weno3", "adr-probA-FirstOrder",
-    "adr-custom-Weno5")]
+    "adr-custom-Weno5", "burgers-outflow-Weno5", "burgers-outflow-FirstOrder")]
 
 
 @pytest.mark.parametrize("case", SAMPLE_CASES, ids=lambda c: c[0])

@@ -203,3 +203,23 @@ def test_adr_probA_cn_implicit(scheme, st, name):
         U = newton_implicit("cn", U, None, 0.05, fj, 1e-5, f_n=fn)
     gold = np.load(os.path.join(TRAJ, f"eigen_2d_advdiffreac_probA_implicit__{name}__gold.npy"))
     assert np.allclose(U, gold, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("scheme,st,name", [(S.FirstOrder, 3, "firstorder"), (S.Weno3, 5, "weno3")])
+def test_burgers_outflow_cn_implicit(scheme, st, name):
+    # tests_cpp/eigen_2d_burgers_outflow_implicit: 20x20 on [-1,1]^2, default
+    # pulse and diffusion, Crank-Nicolson dt 0.01 x 200, Newton tolerance
+    # 1e-5, final state compared at rtol 1e-10, atol 1e-12 (compare.py)
+    mesh = pda.create_structured_mesh((20, 20), (-1.0, 1.0, -1.0, 1.0), st)
+    prob = pda.create_problem(mesh, pda.AdvectionDiffusion2d.BurgersOutflow, scheme, 1)
+    U = np.zeros(prob.totalDofStencilMesh())
+    assert pda.lib().pda_initial_condition_host(prob.handle, U.ctypes.data_as(ctypes.c_void_p)) == 0
+    orc = pyoracle.Oracle()
+    ms = orc.from_mesh(mesh)
+    pb = orc.problem(5, int(pda.AdvectionDiffusion2d.BurgersOutflow), int(scheme), [1e-5])
+    fj = lambda y: orc.velocity_and_jacobian(ms, pb, y)  # noqa: E731
+    for _ in range(200):
+        fn = orc.velocity(ms, pb, U)
+        U = newton_implicit("cn", U, None, 0.01, fj, 1e-5, f_n=fn)
+    gold = np.load(os.path.join(TRAJ, f"eigen_2d_burgers_outflow_implicit__{name}__gold.npy"))
+    assert np.allclose(U, gold, rtol=1e-10, atol=1e-12)
