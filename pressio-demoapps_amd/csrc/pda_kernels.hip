@@ -422,6 +422,240 @@ int jac_weno_staged(const View &v, int w, const double *U, double *V, double *J,
   return w == 2 ? launch_jac_weno_staged<Ph, 2>(v, U, V, J, s) : launch_jac_weno_staged<Ph, 3>(v, U, V, J, s);
 }
 
+// ------------------------- WENO Jacobian of a 2D box, each face once
+// The staged kernel above evaluates every face twice, once for each of its
+// two cells.  For the cells of a structured 2D box (natural order, stencils
+// inside the grid, no near-boundary row) this kernel gives each lane one
+// face of a TX x TY tile of cells instead: its reconstruction gradients,
+// flux Jacobians and 2W blocks are formed once and added to both cells' rows
+// staged in LDS, the cell on the face's plus side taking +h (the face is its
+// left face), the minus side -h (its right face), as
+// impl/mixin_directional_flux_balance_jacobian.hpp:142-284 sums them.  A
+// tile of TX x TY cells has (TX+1) TY x-faces and TX (TY+1) y-faces (those
+// on the tile's edges are also evaluated by the neighbouring tile, for its
+// own cells).  Per row the stage receives, as in the staged kernel: phase 1
+// stores (left face: offsets -W..-1, right face: 1..W, with the family's
+// first-neighbour sources at -+1), phase 2 adds (left face: 1..W-1, right
+// face: -W+1..-1), then the diagonal block and the velocity from the four
+// faces one at a time (x left, x right, y left, y right), then the row's
+// sources; no two lanes write one address in the same phase.  In the box's
+// natural order a tile row's TX rows are consecutive in the CSR, and
+// where no stencil of the tile wraps around a periodic edge the sorted CSR
+// blocks of a row are the y neighbours below (offsets -W..-1), the x
+// neighbours and the cell (-W..W), the y neighbours above (1..W); the
+// tiles at a periodic edge take the stencil cells from the mesh graph and
+// the block slots from the rows' slot tables, so a fully periodic grid is
+// covered whole.
+template <class Ph> struct WenoTile {
+  static constexpr int TX = 8, TY = 3; // 27 x-faces + 32 y-faces = 59 lanes
+};
+
+template <int W> __device__ __forceinline__ int tile_slot(int a, int o) {
+  return a == 0 ? 2 * W + o : (o < 0 ? W + o : 3 * W + o);
+}
+
+// LDS of a tile: stage[NR * PER], vst[NR * M]
+template <class Ph, int W> struct WenoTileLds {
+  static constexpr int NR = WenoTile<Ph>::TX * WenoTile<Ph>::TY, M = Ph::M;
+  static constexpr int DOUBLES = NR * M * (1 + 4 * W) * M + NR * M;
+};
+
+template <class Ph, int W, bool WV>
+__device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B, int64_t bid,
+                                                    const double *__restrict__ U, double *__restrict__ V,
+                                                    double *__restrict__ J, double *stage) {
+  static_assert(Ph::DIM == 2, "2D box");
+  constexpr int M = Ph::M, TX = WenoTile<Ph>::TX, TY = WenoTile<Ph>::TY, NR = TX * TY;
+  constexpr int NB = 1 + 4 * W, RS = NB * M, PER = M * RS, G = 2 * W;
+  constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
+  static_assert(NXF + NYF <= 64, "one face per lane");
+  double *vst = stage + NR * PER;
+  const int lane = threadIdx.x;
+  const int64_t nx = B.n[0];
+  const int64_t bw = B.hi[0] - B.lo[0];
+  const int64_t ntx = (bw + TX - 1) / TX;
+  const int64_t x0 = B.lo[0] + (bid % ntx) * TX, y0 = B.lo[1] + (bid / ntx) * TY;
+  const int nax = int(B.hi[0] - x0 < TX ? B.hi[0] - x0 : TX);
+  const int nay = int(B.hi[1] - y0 < TY ? B.hi[1] - y0 : TY);
+  // a tile some stencil of which wraps around a periodic edge
+  const bool edge = x0 < W || x0 + nax + W > nx || y0 < W || y0 + nay + W > B.n[1];
+  // this lane's face: axis a, tile rows on its minus (rL) and plus (rR)
+  // sides (-1 outside the tile's active cells), its minus-side cell cm (a
+  // tile cell, or the first tile cell's minus neighbour from the graph)
+  int a = 0, rL = -1, rR = -1;
+  int64_t cm = 0;
+  if (lane < NXF) {
+    const int cj = lane / (TX + 1), fi = lane % (TX + 1);
+    if (cj < nay && fi <= nax) {
+      rL = fi >= 1 ? cj * TX + fi - 1 : -1;
+      rR = fi < nax ? cj * TX + fi : -1;
+      const int64_t r0 = (y0 + cj) * nx + x0;
+      cm = fi >= 1 || !edge ? r0 + fi - 1 : int64_t(P.graph[r0 * P.ncols + col_at<2>(0, -1)]);
+    }
+  } else if (lane < NXF + NYF) {
+    a = 1;
+    const int l = lane - NXF, fj = l / TX, ci = l % TX;
+    if (ci < nax && fj <= nay) {
+      rL = fj >= 1 ? (fj - 1) * TX + ci : -1;
+      rR = fj < nay ? fj * TX + ci : -1;
+      const int64_t r0 = y0 * nx + x0 + ci;
+      cm = fj >= 1 || !edge ? r0 + (fj - 1) * nx : int64_t(P.graph[r0 * P.ncols + col_at<2>(1, -1)]);
+    }
+  }
+  const bool act = rL >= 0 || rR >= 0;
+  const double h = a == 0 ? P.hinv[0] : P.hinv[1];
+  // the CSR slot tables of the two rows (row = cell on a full mesh)
+  const int64_t gL = (y0 + (rL >= 0 ? rL : 0) / TX) * nx + x0 + (rL >= 0 ? rL : 0) % TX;
+  const int64_t gR = (y0 + (rR >= 0 ? rR : 0) / TX) * nx + x0 + (rR >= 0 ? rR : 0) % TX;
+  const uint8_t *slL = P.slot + gL * P.npos, *slR = P.slot + gR * P.npos;
+  // block slot of offset o along a (o != 0) in the row's CSR
+  auto slotL = [&](int o) { return edge ? int(slL[col_at<2>(a, o)]) : tile_slot<W>(a, o); };
+  auto slotR = [&](int o) { return edge ? int(slR[col_at<2>(a, o)]) : tile_slot<W>(a, o); };
+  double gm[M][G], gp[M][G], Jn[M * M], Jp[M * M];
+  double fm[M], fp[M], F[M];
+  if (act) {
+    {
+      double q[M][G];
+      const int32_t *grm = P.graph + cm * P.ncols;
+      const int64_t stride = a == 0 ? 1 : nx;
+#pragma unroll
+      for (int i = 0; i < G; ++i) {
+        const int o = i - W + 1;
+        const int64_t c = o == 0 ? cm : (edge ? int64_t(grm[col_at<2>(a, o)]) : cm + o * stride);
+        double u[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) u[k] = U[c * M + k];
+#pragma unroll
+        for (int k = 0; k < M; ++k) q[k][i] = perm_pick<Ph>([&](int x) { return u[x]; }, a, k);
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        if constexpr (W == 2) phys::weno3_grad(q[k], fm[k], fp[k], gm[k], gp[k]);
+        else phys::weno5_grad(q[k], fm[k], fp[k], gm[k], gp[k]);
+      }
+    }
+    Ph::template fjac<0>(P, fm, fp, Jn, Jp);
+    if constexpr (WV) Ph::template flux<0>(P, fm, fp, F);
+  }
+  // the face math and the block products apart (see k_jac_weno_staged)
+  wave_sync();
+  double *stL = stage + (rL >= 0 ? rL : 0) * PER, *stR = stage + (rR >= 0 ? rR : 0) * PER;
+  // phase 1: as the plus cell's left face, offsets -W..-1 (stencil points
+  // 0..W-1); as the minus cell's right face, offsets 1..W (points W..2W-1)
+#pragma unroll
+  for (int m = 0; m < W; ++m) {
+    double b[M][M];
+    if (rR >= 0) {
+      face_block<M, G>(h, Jn, Jp, gm, gp, m, m, false, b);
+      if (m == W - 1) {
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+#pragma unroll
+          for (int j = 0; j < M; ++j) b[k][j] += Sources<Ph>::offdiag(P, false, a, k, j);
+      }
+      put_perm<Ph, RS, false>(stR + slotR(m - W) * M, a, b);
+    }
+    if (rL >= 0) {
+      face_block<M, G>(-h, Jn, Jp, gm, gp, W + m, W + m, false, b);
+      if (m == 0) {
+#pragma unroll
+        for (int k = 0; k < M; ++k)
+#pragma unroll
+          for (int j = 0; j < M; ++j) b[k][j] += Sources<Ph>::offdiag(P, false, a, k, j);
+      }
+      put_perm<Ph, RS, false>(stL + slotL(m + 1) * M, a, b);
+    }
+  }
+  wave_sync();
+  // phase 2: plus cell's offsets 1..W-1 (points W+1..2W-1), minus cell's
+  // offsets -W+1..-1 (points 0..W-2)
+#pragma unroll
+  for (int m = 0; m + 1 < W; ++m) {
+    double b[M][M];
+    if (rR >= 0) {
+      face_block<M, G>(h, Jn, Jp, gm, gp, W + 1 + m, W + 1 + m, false, b);
+      put_perm<Ph, RS, true>(stR + slotR(1 + m) * M, a, b);
+    }
+    if (rL >= 0) {
+      face_block<M, G>(-h, Jn, Jp, gm, gp, m, m, false, b);
+      put_perm<Ph, RS, true>(stL + slotL(m - W + 1) * M, a, b);
+    }
+  }
+  // diagonal blocks and velocity: x faces as left faces (store), x faces as
+  // right faces, y faces as left faces, y faces as right faces (add)
+#pragma unroll
+  for (int ph = 0; ph < 4; ++ph) {
+    wave_sync();
+    const int pa = ph >> 1;
+    const bool plus = (ph & 1) == 0; // the face is the left face of its plus cell
+    const int rr = plus ? rR : rL;
+    if (act && a == pa && rr >= 0) {
+      double b[M][M];
+      const double sh = plus ? h : -h;
+      const int i = plus ? W : W - 1;
+      face_block<M, G>(sh, Jn, Jp, gm, gp, i, i, false, b);
+      double *dst = stage + rr * PER + (edge ? int(plus ? slR[0] : slL[0]) : 2 * W) * M;
+      if (ph == 0) put_perm<Ph, RS, false>(dst, a, b);
+      else put_perm<Ph, RS, true>(dst, a, b);
+      if constexpr (WV) {
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const double v = sh * perm_pick<Ph>([&](int x) { return F[x]; }, a, k);
+          vst[rr * M + k] = (ph == 0 ? 0. : vst[rr * M + k]) + v;
+        }
+      }
+    }
+  }
+  wave_sync();
+  // the rows' sources, one lane per row
+  if (lane < NR && lane % TX < nax && lane / TX < nay) {
+    const int64_t r = (y0 + lane / TX) * nx + x0 + lane % TX;
+    double *st = stage + lane * PER;
+    const int s0 = edge ? int(P.slot[r * P.npos]) : 2 * W;
+    double self[M][M], acc[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      acc[k] = WV ? vst[lane * M + k] : 0.;
+#pragma unroll
+      for (int j = 0; j < M; ++j) self[k][j] = st[k * RS + s0 * M + j];
+    }
+    Sources<Ph>::apply(P, r, false, U, int32_t(r), acc, &self[0][0]);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+#pragma unroll
+      for (int j = 0; j < M; ++j) st[k * RS + s0 * M + j] = self[k][j];
+      if constexpr (WV) V[r * M + k] = acc[k];
+    }
+  }
+  wave_sync();
+  // each tile row: nax consecutive CSR rows, one contiguous range
+  for (int cj = 0; cj < nay; ++cj) {
+    const int64_t r0 = (y0 + cj) * nx + x0;
+    double *dst = J + P.rowbase[r0];
+    const double *src = stage + cj * TX * PER;
+    const int tot = nax * PER;
+    for (int e = lane; e < tot; e += 64) store_stream(dst + e, src[e]);
+  }
+}
+
+template <class Ph, int W, bool WV>
+__global__ void __launch_bounds__(64) k_jac_weno_tiled(View P, Box B, const double *__restrict__ U,
+                                                         double *__restrict__ V, double *__restrict__ J) {
+  __shared__ double smem[WenoTileLds<Ph, W>::DOUBLES];
+  jac_weno_tiled_body<Ph, W, WV>(P, B, int64_t(blockIdx.x), U, V, J, smem);
+}
+
+template <class Ph, int W>
+int launch_jac_weno_tiled(const View &v, const Box &B, const double *U, double *V, double *J, hipStream_t s) {
+  constexpr int TX = WenoTile<Ph>::TX, TY = WenoTile<Ph>::TY;
+  const int64_t ntx = (B.hi[0] - B.lo[0] + TX - 1) / TX, nty = (B.hi[1] - B.lo[1] + TY - 1) / TY;
+  const unsigned nb = unsigned(ntx * nty);
+  if (V) hipLaunchKernelGGL((k_jac_weno_tiled<Ph, W, true>), dim3(nb), dim3(64), 0, s, v, B, U, V, J);
+  else hipLaunchKernelGGL((k_jac_weno_tiled<Ph, W, false>), dim3(nb), dim3(64), 0, s, v, B, U, V, J);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
 // ------------------------------------------------- Jacobian action (SpMM)
 // AJ = J * B.  One thread per (scalar row, column of B).
 __global__ void k_apply(const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
@@ -583,7 +817,49 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
     }
     if (!p.view.dup && p.w > 1 && p.family != PDA_ADR2D) {
       // WENO Jacobian: staged inner rows, then the near-boundary rows (first
-      // order with BC factors) with the generic row code
+      // order with BC factors) with the generic row code; for the 2D
+      // two- and three-dof families, the box's rows with each face once
+      // (when the box holds no near-boundary row) and the staged kernel on
+      // the rows outside it
+      if (p.dim == 2 && (p.family == PDA_BURGERS2D || p.family == PDA_SWE2D) && p.box.valid &&
+          p.box.cells() > 0) {
+        // tiles over the whole grid when it is periodic along both axes,
+        // else over the box when it holds no near-boundary row (its
+        // non-periodic sides at least the mesh stencil's half width in)
+        const int wm = (p.mesh->stencil - 1) / 2;
+        const int per = p.mesh->periodic_mask & 3;
+        Box tb = p.box;
+        bool ok = true;
+        if (per == 3) {
+          for (int ax = 0; ax < 2; ++ax) {
+            tb.lo[ax] = 0;
+            tb.hi[ax] = tb.n[ax];
+          }
+        } else if (per == 0) {
+          for (int ax = 0; ax < 2; ++ax) ok = ok && p.box.lo[ax] >= wm;
+        } else {
+          ok = false;
+        }
+        if (ok) {
+          const View vb = generic_view(p, nullptr, 0);
+          int rc = p.family == PDA_BURGERS2D
+                       ? (p.w == 2 ? launch_jac_weno_tiled<PhBurgers, 2>(vb, tb, U, V, J, s)
+                                   : launch_jac_weno_tiled<PhBurgers, 3>(vb, tb, U, V, J, s))
+                       : (p.w == 2 ? launch_jac_weno_tiled<PhSwe, 2>(vb, tb, U, V, J, s)
+                                   : launch_jac_weno_tiled<PhSwe, 3>(vb, tb, U, V, J, s));
+          if (rc) return rc;
+          if (per == 0 && int64_t(p.h_rem_rows.size()) > p.n_near) {
+            // the rows outside the box that are not near-boundary rows (the
+            // staged kernel skips those)
+            const View vr = generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size()));
+            rc = p.family == PDA_BURGERS2D ? jac_weno_staged<PhBurgers>(vr, p.w, U, V, J, s)
+                                           : jac_weno_staged<PhSwe>(vr, p.w, U, V, J, s);
+            if (rc) return rc;
+          }
+          if (p.n_near == 0) return PDA_OK;
+          return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);
+        }
+      }
       const View v = generic_view(p, nullptr, 0);
       int rc;
       switch (p.family) {

@@ -9,6 +9,8 @@ Cases:
   euler2d-weno5   2D Euler Riemann IC2, 1024^2, WENO5 (13 4x4 blocks per row)
   euler2d-weno3   2D Euler Riemann IC2, 1024^2, WENO3
   burgers-weno5   2D viscous Burgers, periodic, 512^2, WENO5 (the implicit workload's Jacobian)
+  burgers-weno3   the same with WENO3
+  burgers-outflow-weno5  2D viscous Burgers, outflow boundaries, 512^2, WENO5
   swe-weno3       shallow water, slip walls, 512^2, WENO3
   euler3d-weno3   3D Euler Sedov, 128^3, WENO3
 """
@@ -26,6 +28,11 @@ CASES = {
                               S.Weno3, 2, None),
     "burgers-weno5": lambda: (create_structured_mesh((512, 512), (-1.0, 1.0, -1.0, 1.0), 7, True),
                               AdvectionDiffusion2d.BurgersPeriodic, S.Weno5, 1, {"diffusion": 5e-3}),
+    "burgers-weno3": lambda: (create_structured_mesh((512, 512), (-1.0, 1.0, -1.0, 1.0), 5, True),
+                              AdvectionDiffusion2d.BurgersPeriodic, S.Weno3, 1, {"diffusion": 5e-3}),
+    "burgers-outflow-weno5": lambda: (create_structured_mesh((512, 512), (-1.0, 1.0, -1.0, 1.0), 7),
+                                      AdvectionDiffusion2d.BurgersOutflow, S.Weno5, 1,
+                                      {"diffusion": 5e-3, "pulseX": 0.3, "pulseY": 0.2}),
     "swe-weno3": lambda: (create_structured_mesh((512, 512), (-5.0, 5.0, -5.0, 5.0), 5), Swe2d.SlipWall, S.Weno3,
                           1, None),
     "euler3d-weno3": lambda: (create_structured_mesh((128, 128, 128), (0.0, 1.2, 0.0, 1.2, 0.0, 1.2), 5),

@@ -56,6 +56,13 @@ BIG_CASES = [
     ("euler2d-periodic-big-Weno3", pda.Euler2d.PeriodicSmooth, S.Weno3, (70, 131), (-1, 1, -1, 1), 5, True, 0, None),
     ("swe-big-FirstOrder", pda.Swe2d.SlipWall, S.FirstOrder, (97, 70), (-5, 5, -5, 5), 7, (), 1,
      {"gravity": 9.8, "coriolis": -3.0}),
+    # WENO Jacobian with each face once over 8 x 3 tiles (partial tiles at
+    # the box's far edges; periodic tiles at the wrap take graph and slot
+    # tables)
+    ("swe-big-Weno5", pda.Swe2d.SlipWall, S.Weno5, (61, 44), (-5, 5, -5, 5), 7, (), 1,
+     {"gravity": 9.8, "coriolis": -3.0}),
+    ("burgers-big-Weno3", pda.AdvectionDiffusion2d.BurgersPeriodic, S.Weno3, (45, 38), (-1, 1, -1, 1), 5,
+     True, 1, {"diffusion": 5e-3}),
     ("burgers-big-Weno5", pda.AdvectionDiffusion2d.BurgersPeriodic, S.Weno5, (130, 66), (-1, 1, -1, 1), 7,
      True, 1, {"diffusion": 5e-3}),
     ("burgers-big-FirstOrder", pda.AdvectionDiffusion2d.BurgersPeriodic, S.FirstOrder, (130, 66), (-1, 1, -1, 1), 3,
