@@ -446,9 +446,53 @@ int jac_weno_staged(const View &v, int w, const double *U, double *V, double *J,
 // tiles at a periodic edge take the stencil cells from the mesh graph and
 // the block slots from the rows' slot tables, so a fully periodic grid is
 // covered whole.
+// LPF lanes share a face: with two, each forms the reconstruction gradients
+// of half the dofs and the matching half of the block columns, the face
+// states are exchanged between the pair and both form the flux Jacobians.
 template <class Ph> struct WenoTile {
-  static constexpr int TX = 8, TY = 3; // 27 x-faces + 32 y-faces = 59 lanes
+  static constexpr int TX = 8, TY = 3, LPF = 1; // 27 x-faces + 32 y-faces = 59 lanes
 };
+template <> struct WenoTile<PhEuler2> {
+  static constexpr int TX = 4, TY = 3, LPF = 2; // (15 + 16) faces x 2 lanes = 62 lanes
+};
+
+// columns j = j0 + v (v < NV) of a face's block at stencil point i, in the
+// axis frame: b[k][v] = sh (Jn[k][j] gm[v][i] + Jp[k][j] gp[v][i]); j0 is 0
+// or NV, chosen by sel
+template <int M, int NV, int G>
+__device__ __forceinline__ void face_cols(double sh, const double (&Jn)[M * M], const double (&Jp)[M * M],
+                                          const double (&gm)[NV][G], const double (&gp)[NV][G], int i, bool sel,
+                                          double (&b)[M][NV]) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int v = 0; v < NV; ++v)
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double jn = sel ? Jn[k * M + NV + v] : Jn[k * M + v];
+      const double jp = sel ? Jp[k * M + NV + v] : Jp[k * M + v];
+      b[k][v] = sh * (jn * gm[v][i] + jp * gp[v][i]);
+    }
+}
+
+// store (or add) block columns j0 + v of the axis frame at dst (row stride
+// RS) in the physical frame: dst[K][p_a(j)] = b[p_a(K)][v]
+template <class Ph, int RS, bool ADD, int NV>
+__device__ __forceinline__ void put_perm_cols(double *dst, int a, int j0, const double (&b)[Ph::M][NV]) {
+  constexpr int M = Ph::M;
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const int j = j0 + v;
+    const int J = a == 0 ? AxisPerm<Ph>::p(j, 0) : (a == 1 ? AxisPerm<Ph>::p(j, 1) : AxisPerm<Ph>::p(j, 2));
+#pragma unroll
+    for (int K = 0; K < M; ++K) {
+      double x = b[AxisPerm<Ph>::p(K, 0)][v];
+      if constexpr (Ph::DIM > 1) x = a == 1 ? b[AxisPerm<Ph>::p(K, 1)][v] : x;
+      if constexpr (Ph::DIM > 2) x = a == 2 ? b[AxisPerm<Ph>::p(K, 2)][v] : x;
+      if constexpr (ADD) dst[K * RS + J] += x;
+      else dst[K * RS + J] = x;
+    }
+  }
+}
 
 template <int W> __device__ __forceinline__ int tile_slot(int a, int o) {
   return a == 0 ? 2 * W + o : (o < 0 ? W + o : 3 * W + o);
@@ -466,11 +510,13 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
                                                     double *__restrict__ J, double *stage) {
   static_assert(Ph::DIM == 2, "2D box");
   constexpr int M = Ph::M, TX = WenoTile<Ph>::TX, TY = WenoTile<Ph>::TY, NR = TX * TY;
+  constexpr int LPF = WenoTile<Ph>::LPF, NV = M / LPF;
   constexpr int NB = 1 + 4 * W, RS = NB * M, PER = M * RS, G = 2 * W;
   constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
-  static_assert(NXF + NYF <= 64, "one face per lane");
+  static_assert(LPF * (NXF + NYF) <= 64 && NV * LPF == M, "LPF lanes per face");
   double *vst = stage + NR * PER;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x, fl = lane / LPF, hv = lane % LPF; // face, half
+  const int j0 = hv * NV;                                           // the lane's first dof
   const int64_t nx = B.n[0];
   const int64_t bw = B.hi[0] - B.lo[0];
   const int64_t ntx = (bw + TX - 1) / TX;
@@ -484,17 +530,17 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   // tile cell, or the first tile cell's minus neighbour from the graph)
   int a = 0, rL = -1, rR = -1;
   int64_t cm = 0;
-  if (lane < NXF) {
-    const int cj = lane / (TX + 1), fi = lane % (TX + 1);
+  if (fl < NXF) {
+    const int cj = fl / (TX + 1), fi = fl % (TX + 1);
     if (cj < nay && fi <= nax) {
       rL = fi >= 1 ? cj * TX + fi - 1 : -1;
       rR = fi < nax ? cj * TX + fi : -1;
       const int64_t r0 = (y0 + cj) * nx + x0;
       cm = fi >= 1 || !edge ? r0 + fi - 1 : int64_t(P.graph[r0 * P.ncols + col_at<2>(0, -1)]);
     }
-  } else if (lane < NXF + NYF) {
+  } else if (fl < NXF + NYF) {
     a = 1;
-    const int l = lane - NXF, fj = l / TX, ci = l % TX;
+    const int l = fl - NXF, fj = l / TX, ci = l % TX;
     if (ci < nax && fj <= nay) {
       rL = fj >= 1 ? (fj - 1) * TX + ci : -1;
       rR = fj < nay ? fj * TX + ci : -1;
@@ -511,29 +557,51 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   // block slot of offset o along a (o != 0) in the row's CSR
   auto slotL = [&](int o) { return edge ? int(slL[col_at<2>(a, o)]) : tile_slot<W>(a, o); };
   auto slotR = [&](int o) { return edge ? int(slR[col_at<2>(a, o)]) : tile_slot<W>(a, o); };
-  double gm[M][G], gp[M][G], Jn[M * M], Jp[M * M];
-  double fm[M], fp[M], F[M];
+  // gm, gp: the lane's NV dofs (axis frame j0..j0+NV-1); fm, fp: all
+  double gm[NV][G], gp[NV][G], Jn[M * M], Jp[M * M];
+  double fm[M] = {}, fp[M] = {}, F[M];
   if (act) {
-    {
-      double q[M][G];
-      const int32_t *grm = P.graph + cm * P.ncols;
-      const int64_t stride = a == 0 ? 1 : nx;
+    double q[NV][G];
+    const int32_t *grm = P.graph + cm * P.ncols;
+    const int64_t stride = a == 0 ? 1 : nx;
 #pragma unroll
-      for (int i = 0; i < G; ++i) {
-        const int o = i - W + 1;
-        const int64_t c = o == 0 ? cm : (edge ? int64_t(grm[col_at<2>(a, o)]) : cm + o * stride);
-        double u[M];
+    for (int i = 0; i < G; ++i) {
+      const int o = i - W + 1;
+      const int64_t c = o == 0 ? cm : (edge ? int64_t(grm[col_at<2>(a, o)]) : cm + o * stride);
+      double u[M];
 #pragma unroll
-        for (int k = 0; k < M; ++k) u[k] = U[c * M + k];
+      for (int k = 0; k < M; ++k) u[k] = U[c * M + k];
 #pragma unroll
-        for (int k = 0; k < M; ++k) q[k][i] = perm_pick<Ph>([&](int x) { return u[x]; }, a, k);
-      }
-#pragma unroll
-      for (int k = 0; k < M; ++k) {
-        if constexpr (W == 2) phys::weno3_grad(q[k], fm[k], fp[k], gm[k], gp[k]);
-        else phys::weno5_grad(q[k], fm[k], fp[k], gm[k], gp[k]);
+      for (int v = 0; v < NV; ++v) {
+        const double x0v = perm_pick<Ph>([&](int x) { return u[x]; }, a, v);
+        q[v][i] = LPF == 1 || hv == 0 ? x0v : perm_pick<Ph>([&](int x) { return u[x]; }, a, (NV + v) % M);
       }
     }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      double um, up;
+      if constexpr (W == 2) phys::weno3_grad(q[v], um, up, gm[v], gp[v]);
+      else phys::weno5_grad(q[v], um, up, gm[v], gp[v]);
+#pragma unroll
+      for (int k = 0; k < M; ++k)
+        if (k == j0 + v) {
+          fm[k] = um;
+          fp[k] = up;
+        }
+    }
+  }
+  if constexpr (LPF == 2) {
+    // the pair's other half of the face states (every lane takes part)
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double tm = __shfl_xor(fm[k], 1), tp = __shfl_xor(fp[k], 1);
+      if (k / NV != hv) {
+        fm[k] = tm;
+        fp[k] = tp;
+      }
+    }
+  }
+  if (act) {
     Ph::template fjac<0>(P, fm, fp, Jn, Jp);
     if constexpr (WV) Ph::template flux<0>(P, fm, fp, F);
   }
@@ -544,26 +612,26 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   // 0..W-1); as the minus cell's right face, offsets 1..W (points W..2W-1)
 #pragma unroll
   for (int m = 0; m < W; ++m) {
-    double b[M][M];
+    double b[M][NV];
     if (rR >= 0) {
-      face_block<M, G>(h, Jn, Jp, gm, gp, m, m, false, b);
+      face_cols<M, NV, G>(h, Jn, Jp, gm, gp, m, hv != 0, b);
       if (m == W - 1) {
 #pragma unroll
         for (int k = 0; k < M; ++k)
 #pragma unroll
-          for (int j = 0; j < M; ++j) b[k][j] += Sources<Ph>::offdiag(P, false, a, k, j);
+          for (int v = 0; v < NV; ++v) b[k][v] += Sources<Ph>::offdiag(P, false, a, k, j0 + v);
       }
-      put_perm<Ph, RS, false>(stR + slotR(m - W) * M, a, b);
+      put_perm_cols<Ph, RS, false, NV>(stR + slotR(m - W) * M, a, j0, b);
     }
     if (rL >= 0) {
-      face_block<M, G>(-h, Jn, Jp, gm, gp, W + m, W + m, false, b);
+      face_cols<M, NV, G>(-h, Jn, Jp, gm, gp, W + m, hv != 0, b);
       if (m == 0) {
 #pragma unroll
         for (int k = 0; k < M; ++k)
 #pragma unroll
-          for (int j = 0; j < M; ++j) b[k][j] += Sources<Ph>::offdiag(P, false, a, k, j);
+          for (int v = 0; v < NV; ++v) b[k][v] += Sources<Ph>::offdiag(P, false, a, k, j0 + v);
       }
-      put_perm<Ph, RS, false>(stL + slotL(m + 1) * M, a, b);
+      put_perm_cols<Ph, RS, false, NV>(stL + slotL(m + 1) * M, a, j0, b);
     }
   }
   wave_sync();
@@ -571,14 +639,14 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   // offsets -W+1..-1 (points 0..W-2)
 #pragma unroll
   for (int m = 0; m + 1 < W; ++m) {
-    double b[M][M];
+    double b[M][NV];
     if (rR >= 0) {
-      face_block<M, G>(h, Jn, Jp, gm, gp, W + 1 + m, W + 1 + m, false, b);
-      put_perm<Ph, RS, true>(stR + slotR(1 + m) * M, a, b);
+      face_cols<M, NV, G>(h, Jn, Jp, gm, gp, W + 1 + m, hv != 0, b);
+      put_perm_cols<Ph, RS, true, NV>(stR + slotR(1 + m) * M, a, j0, b);
     }
     if (rL >= 0) {
-      face_block<M, G>(-h, Jn, Jp, gm, gp, m, m, false, b);
-      put_perm<Ph, RS, true>(stL + slotL(m - W + 1) * M, a, b);
+      face_cols<M, NV, G>(-h, Jn, Jp, gm, gp, m, hv != 0, b);
+      put_perm_cols<Ph, RS, true, NV>(stL + slotL(m - W + 1) * M, a, j0, b);
     }
   }
   // diagonal blocks and velocity: x faces as left faces (store), x faces as
@@ -590,14 +658,14 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
     const bool plus = (ph & 1) == 0; // the face is the left face of its plus cell
     const int rr = plus ? rR : rL;
     if (act && a == pa && rr >= 0) {
-      double b[M][M];
+      double b[M][NV];
       const double sh = plus ? h : -h;
       const int i = plus ? W : W - 1;
-      face_block<M, G>(sh, Jn, Jp, gm, gp, i, i, false, b);
+      face_cols<M, NV, G>(sh, Jn, Jp, gm, gp, i, hv != 0, b);
       double *dst = stage + rr * PER + (edge ? int(plus ? slR[0] : slL[0]) : 2 * W) * M;
-      if (ph == 0) put_perm<Ph, RS, false>(dst, a, b);
-      else put_perm<Ph, RS, true>(dst, a, b);
-      if constexpr (WV) {
+      if (ph == 0) put_perm_cols<Ph, RS, false, NV>(dst, a, j0, b);
+      else put_perm_cols<Ph, RS, true, NV>(dst, a, j0, b);
+      if (WV && hv == 0) {
 #pragma unroll
         for (int k = 0; k < M; ++k) {
           const double v = sh * perm_pick<Ph>([&](int x) { return F[x]; }, a, k);
@@ -654,6 +722,11 @@ int launch_jac_weno_tiled(const View &v, const Box &B, const double *U, double *
   else hipLaunchKernelGGL((k_jac_weno_tiled<Ph, W, false>), dim3(nb), dim3(64), 0, s, v, B, U, V, J);
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
+}
+
+template <class Ph>
+int jac_weno_tiled(const View &v, const Box &B, int w, const double *U, double *V, double *J, hipStream_t s) {
+  return w == 2 ? launch_jac_weno_tiled<Ph, 2>(v, B, U, V, J, s) : launch_jac_weno_tiled<Ph, 3>(v, B, U, V, J, s);
 }
 
 // ------------------------------------------------- Jacobian action (SpMM)
@@ -821,8 +894,8 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
       // two- and three-dof families, the box's rows with each face once
       // (when the box holds no near-boundary row) and the staged kernel on
       // the rows outside it
-      if (p.dim == 2 && (p.family == PDA_BURGERS2D || p.family == PDA_SWE2D) && p.box.valid &&
-          p.box.cells() > 0) {
+      if (p.dim == 2 && (p.family == PDA_BURGERS2D || p.family == PDA_SWE2D || p.family == PDA_EULER2D) &&
+          p.box.valid && p.box.cells() > 0) {
         // tiles over the whole grid when it is periodic along both axes,
         // else over the box when it holds no near-boundary row (its
         // non-periodic sides at least the mesh stencil's half width in)
@@ -842,18 +915,17 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
         }
         if (ok) {
           const View vb = generic_view(p, nullptr, 0);
-          int rc = p.family == PDA_BURGERS2D
-                       ? (p.w == 2 ? launch_jac_weno_tiled<PhBurgers, 2>(vb, tb, U, V, J, s)
-                                   : launch_jac_weno_tiled<PhBurgers, 3>(vb, tb, U, V, J, s))
-                       : (p.w == 2 ? launch_jac_weno_tiled<PhSwe, 2>(vb, tb, U, V, J, s)
-                                   : launch_jac_weno_tiled<PhSwe, 3>(vb, tb, U, V, J, s));
+          int rc = p.family == PDA_BURGERS2D ? jac_weno_tiled<PhBurgers>(vb, tb, p.w, U, V, J, s)
+                   : p.family == PDA_SWE2D   ? jac_weno_tiled<PhSwe>(vb, tb, p.w, U, V, J, s)
+                                             : jac_weno_tiled<PhEuler2>(vb, tb, p.w, U, V, J, s);
           if (rc) return rc;
           if (per == 0 && int64_t(p.h_rem_rows.size()) > p.n_near) {
             // the rows outside the box that are not near-boundary rows (the
             // staged kernel skips those)
             const View vr = generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size()));
             rc = p.family == PDA_BURGERS2D ? jac_weno_staged<PhBurgers>(vr, p.w, U, V, J, s)
-                                           : jac_weno_staged<PhSwe>(vr, p.w, U, V, J, s);
+                 : p.family == PDA_SWE2D   ? jac_weno_staged<PhSwe>(vr, p.w, U, V, J, s)
+                                           : jac_weno_staged<PhEuler2>(vr, p.w, U, V, J, s);
             if (rc) return rc;
           }
           if (p.n_near == 0) return PDA_OK;
