@@ -338,7 +338,7 @@ def main():
             torch.cuda.synchronize()
             best = min(best, (time.perf_counter() - t) / 12 * 1e6)
         return best
-    cands = sorted({3, 4, 6, 8, 10, 12, 16, rps})
+    cands = sorted({4, 6, 8, 10, 12, 16, 20, 24, 32, rps})
     ctx = {h: ctx_us(h) for h in cands}
     rps = min(ctx, key=ctx.get)
     probA.set_velocity_strip_height(rps)

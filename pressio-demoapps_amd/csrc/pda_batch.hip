@@ -55,7 +55,7 @@ int build_tables(pda_batch_s &b, int rps) {
     I.off = b.off[i];
     I.blk0 = blk;
     I.rps = rps;
-    const int64_t nb = velocity_march_blocks(I.B, rps);
+    const int64_t nb = velocity_march_blocks(I.B, 2, b.w, rps);
     b.h_inst.push_back(I);
     for (int64_t q = 0; q < nb; ++q) b.h_blk.push_back(int32_t(i));
     blk += nb;

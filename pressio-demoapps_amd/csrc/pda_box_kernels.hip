@@ -4,9 +4,10 @@
 // quantities as the per-row kernels (same face states, fluxes and flux
 // Jacobians, same per-cell accumulation order; values agree to rounding)
 // but are laid out for the hardware:
-//   - velocity: one wavefront marches a 63-cell row segment up a strip of
-//     rows, evaluating each face once (x faces shared by lane shuffles, y
-//     faces carried in registers);
+//   - velocity: one wavefront marches a row segment up a strip of rows,
+//     evaluating each face once (x faces shared by lane shuffles, y faces
+//     carried in registers); in 2D each cell is reconstructed once per axis
+//     from rows held in registers;
 //   - first-order Jacobian: the blocks are assembled in LDS in CSR order and
 //     leave as one contiguous run per row segment (coalesced 16-byte stores);
 //     for 2D Euler two lanes per cell evaluate each face once.
@@ -154,6 +155,180 @@ __device__ __forceinline__ void vel_march_body(const View &P, const Box &B, int 
   }
 }
 
+// ------------------------------------------ velocity 2D, cell-centred march
+// A wavefront holds 64 consecutive columns of a grid row, the 64 - 2W box
+// cells it owns and W columns on each side, and marches up a strip of rows
+// with the 2W - 1 rows a y reconstruction reads held in registers: one row
+// is loaded per step.  Each lane reconstructs its own cell once per axis,
+// the cell's low-side and high-side values sharing the WENO windows
+// (weno5_cell_1div).  The x face on the low side of lane l's cell takes the
+// high-side value of lane l - 1 and the low-side value of lane l, the x
+// stencil being the neighbouring lanes' row values (DPP wavefront shifts,
+// no LDS); the face on the high side is lane l + 1's.  The y face above row r takes the high-side
+// value of row r, carried from the previous step, and the low-side value of
+// row r + 1.  Every face value equals the face-by-face march's bit for bit.
+template <int W> __host__ __device__ constexpr int vcc_width() { return 64 - 2 * W; }
+
+// state of cell (i, j) of a 2D grid, each index possibly outside the grid
+// and mapped along its axis as load_cell_mapped maps it
+template <int M>
+__device__ __forceinline__ void load_cell_mapped2(const View &P, const Box &B, const double *__restrict__ U,
+                                                  int64_t i, int64_t j, double *q) {
+  double sg[M];
+#pragma unroll
+  for (int k = 0; k < M; ++k) sg[k] = 1.;
+  int64_t c[2] = {i, j};
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t n = B.n[a];
+    int side = -1;
+    if (c[a] < 0) {
+      if (B.mode[a] == 1) c[a] += n;
+      else { c[a] = -1 - c[a]; side = 0; }
+    } else if (c[a] >= n) {
+      if (B.mode[a] == 1) c[a] -= n;
+      else { c[a] = 2 * n - 1 - c[a]; side = 1; }
+    }
+    if (side >= 0) {
+#pragma unroll
+      for (int k = 0; k < M; ++k) sg[k] *= P.gsign[a][side][k];
+    }
+  }
+  const int64_t cell = c[1] * B.n[0] + c[0];
+#pragma unroll
+  for (int k = 0; k < M; ++k) q[k] = sg[k] * U[cell * M + k];
+}
+
+// value of lane l - 1 (lane_prev) or l + 1 (lane_next) of the wavefront,
+// by DPP wavefront shifts of the two halves; lane 0 (lane 63) gets 0
+__device__ __forceinline__ double lane_dpp(double v, int ctrl_is_next) {
+  const long long b = __double_as_longlong(v);
+  int lo, hi;
+  if (ctrl_is_next) {
+    lo = __builtin_amdgcn_update_dpp(0, int(b), 0x130, 0xf, 0xf, false);        // wave_shl:1
+    hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x130, 0xf, 0xf, false);
+  } else {
+    lo = __builtin_amdgcn_update_dpp(0, int(b), 0x138, 0xf, 0xf, false);        // wave_shr:1
+    hi = __builtin_amdgcn_update_dpp(0, int(b >> 32), 0x138, 0xf, 0xf, false);
+  }
+  return __longlong_as_double((long long)(unsigned(lo)) | ((long long)hi << 32));
+}
+__device__ __forceinline__ double lane_prev(double v) { return lane_dpp(v, 0); }
+__device__ __forceinline__ double lane_next(double v) { return lane_dpp(v, 1); }
+
+// both face values of the middle cell of s[0 .. 2W-2] (W = 1: the value)
+template <int W>
+__device__ __forceinline__ void recon_cell(const double *s, double &hi, double &lo) {
+  if constexpr (W == 1) {
+    hi = s[0];
+    lo = s[0];
+  } else if constexpr (W == 2) {
+    phys::weno3_cell_1div(s[0], s[1], s[2], hi, lo);
+  } else {
+    phys::weno5_cell_1div(s[0], s[1], s[2], s[3], s[4], hi, lo);
+  }
+}
+
+template <class Ph, int W>
+__device__ __forceinline__ void vel_cc_body(const View &P, const Box &B, int rows_per_strip, int64_t bid,
+                                            const double *__restrict__ U, double *__restrict__ V) {
+  static_assert(Ph::DIM == 2, "2D march");
+  constexpr int M = Ph::M, NC = vcc_width<W>(), NR = 2 * W - 1;
+  const int lane = threadIdx.x;
+  const int64_t nx = B.n[0];
+  const int64_t bw = B.hi[0] - B.lo[0];
+  const int64_t nseg = (bw + NC - 1) / NC;
+  const int64_t seg = bid % nseg, strip = bid / nseg;
+  const int64_t iseg = B.lo[0] + seg * NC;
+  const int ncell = int(bw - seg * NC < NC ? bw - seg * NC : NC);
+  const int64_t j0 = B.lo[1] + strip * rows_per_strip;
+  const int64_t j1 = j0 + rows_per_strip < B.hi[1] ? j0 + rows_per_strip : B.hi[1];
+  // lane l holds column iseg - W + l; lanes past the last column that the
+  // segment's cells read hold that column again
+  const int lc = lane < ncell + 2 * W - 1 ? lane : ncell + 2 * W - 1;
+  const int64_t i = iseg - W + lc;
+  const bool own = lane >= W && lane < W + ncell;
+  const double hx = P.hinv[0], hy = P.hinv[1];
+  // win[p] = row r + 2 - W + p at step r (W = 1: row r + 1); nxt = the
+  // row that enters the window at the next step, loaded one step ahead
+  double win[NR][M], nxt[M];
+#pragma unroll
+  for (int p = 0; p < NR; ++p) load_cell_mapped2<M>(P, B, U, i, j0 - W + p, win[p]);
+  load_cell_mapped2<M>(P, B, U, i, j0 - 1 + W, nxt);
+  double qhi[M], fb[M];  // high-side y value of row r; flux of the face below row r
+  for (int64_t r = j0 - 2; r < j1; ++r) {
+    if (r > j0 - 2) {
+#pragma unroll
+      for (int p = 0; p + 1 < NR; ++p)
+#pragma unroll
+        for (int k = 0; k < M; ++k) win[p][k] = win[p + 1][k];
+#pragma unroll
+      for (int k = 0; k < M; ++k) win[NR - 1][k] = nxt[k];
+      if (r + 1 < j1) load_cell_mapped2<M>(P, B, U, i, r + 1 + W, nxt);
+    }
+    // y face values of cell (i, r + 1)
+    double yhi[M], ylo[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      double s[NR];
+#pragma unroll
+      for (int p = 0; p < NR; ++p) s[p] = win[p][k];
+      recon_cell<W>(s, yhi[k], ylo[k]);
+    }
+    if (r >= j0 - 1) {
+      double fa[M];  // the face above row r
+      Ph::template flux<1>(P, qhi, ylo, fa);
+      if (r >= j0) {
+        // x face values of cell (i, r), the stencil from the lanes beside
+        double xhi[M], xlo[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          double s[NR];
+          s[W - 1] = W >= 2 ? win[W >= 2 ? W - 2 : 0][k] : qhi[k];
+#pragma unroll
+          for (int o = 1; o < W; ++o) {
+            s[W - 1 - o] = lane_prev(s[W - o]);
+            s[W - 1 + o] = lane_next(s[W - 2 + o]);
+          }
+          recon_cell<W>(s, xhi[k], xlo[k]);
+        }
+        double ql[M], fx[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) ql[k] = lane_prev(xhi[k]);
+        Ph::template flux<0>(P, ql, xlo, fx);
+        double acc[M];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+          const double fr = lane_next(fx[k]);
+          acc[k] = 0. + hx * (fx[k] - fr);
+          acc[k] += hy * (fb[k] - fa[k]);
+        }
+        if (own) {
+          const int64_t c = r * nx + i;
+          Sources<Ph>::apply(P, c, false, U, int32_t(c), acc, nullptr);
+#pragma unroll
+          for (int k = 0; k < M; ++k) V[c * M + k] = acc[k];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < M; ++k) fb[k] = fa[k];
+    }
+#pragma unroll
+    for (int k = 0; k < M; ++k) qhi[k] = yhi[k];
+  }
+}
+
+// waves per SIMD the 2D march is compiled for: 3 (up to 168 registers)
+// where the rows of a WENO window of three or more dofs do not fit 128
+// registers without spilling
+template <class Ph, int W> constexpr int vcc_waves() { return W >= 2 && Ph::M >= 3 ? 3 : 4; }
+
+template <class Ph, int W>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(vcc_waves<Ph, W>()))) k_vel_cc(View P, Box B, int rows_per_strip,
+                                                  const double *__restrict__ U, double *__restrict__ V) {
+  vel_cc_body<Ph, W>(P, B, rows_per_strip, xcd_contiguous(int64_t(blockIdx.x), int64_t(gridDim.x)), U, V);
+}
+
 template <class Ph, int W>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Ph::DIM == 3 ? 2 : 4))) k_vel_march(View P, Box B, int rows_per_strip,
                                                      const double *__restrict__ U,
@@ -165,12 +340,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Ph::DIM
 // grid, parameters and state slice) in one launch: blk_inst maps a
 // workgroup to its instance, whose workgroups are numbered from blk0
 template <class Ph, int W>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) k_vel_march2d_batch(const BatchInst *__restrict__ inst,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(vcc_waves<Ph, W>()))) k_vel_march2d_batch(const BatchInst *__restrict__ inst,
                                                            const int32_t *__restrict__ blk_inst,
                                                            const double *__restrict__ U,
                                                            double *__restrict__ V) {
   const BatchInst &I = inst[blk_inst[blockIdx.x]];
-  vel_march_body<Ph, W>(I.P, I.B, I.rps, int64_t(blockIdx.x) - I.blk0, U + I.off, V + I.off);
+  vel_cc_body<Ph, W>(I.P, I.B, I.rps, int64_t(blockIdx.x) - I.blk0, U + I.off, V + I.off);
 }
 
 // ------------------------------------------------- first-order Jacobian 2D
@@ -514,14 +689,18 @@ bool has_box_velocity(const pda_problem_s &p) {
 template <class Ph>
 static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
   const Box &B = p.vbox;
-  const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
-  const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
-  int64_t rps = velocity_strip_height(B);
+  int64_t rps = velocity_strip_height(B, Ph::DIM, p.w);
   if (p.vel_rps > 0) rps = p.vel_rps; // measured by pda_problem_autotune
-  const dim3 grid{unsigned(nseg * ((bh + rps - 1) / rps) * planes), 1, 1};
-  if (p.w == 1) hipLaunchKernelGGL((k_vel_march<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
-  else if (p.w == 2) hipLaunchKernelGGL((k_vel_march<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
-  else hipLaunchKernelGGL((k_vel_march<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  const dim3 grid{unsigned(velocity_march_blocks(B, Ph::DIM, p.w, int(rps))), 1, 1};
+  if constexpr (Ph::DIM == 2) {
+    if (p.w == 1) hipLaunchKernelGGL((k_vel_cc<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+    else if (p.w == 2) hipLaunchKernelGGL((k_vel_cc<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+    else hipLaunchKernelGGL((k_vel_cc<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  } else {
+    if (p.w == 1) hipLaunchKernelGGL((k_vel_march<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+    else if (p.w == 2) hipLaunchKernelGGL((k_vel_march<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+    else hipLaunchKernelGGL((k_vel_march<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+  }
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
 }
@@ -578,15 +757,21 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
   return PDA_E_UNSUPPORTED;
 }
 
+// cells of a row that one velocity wavefront owns
+static int64_t velocity_segment(int dim, int w) {
+  return dim == 2 ? (w == 1 ? vcc_width<1>() : (w == 2 ? vcc_width<2>() : vcc_width<3>())) : VSEG;
+}
+
 // default strip height of the single-problem velocity march.  Each strip
-// evaluates one extra row of y faces, so taller strips do less work but
+// evaluates extra rows of y values, so taller strips do less work but
 // give fewer waves; 2D grids aim at ~3 waves per SIMD (1024^2: 6 rows,
-// 2,907 waves on 1,024 SIMDs), 3D grids at ~20 strips per plane (the
+// 3,078 waves on 1,024 SIMDs), 3D grids at ~20 strips per plane (the
 // measured best: 12 rows at 256^3, 5 at 128^3).
-int64_t velocity_strip_height(const Box &B) {
-  const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
-  const int64_t bh = B.hi[1] - B.lo[1], planes = B.hi[2] - B.lo[2];
-  int64_t rps = planes > 1 ? bh / 20 : (bh * nseg + 3071) / 3072;
+int64_t velocity_strip_height(const Box &B, int dim, int w) {
+  const int64_t seg = velocity_segment(dim, w);
+  const int64_t nseg = (B.hi[0] - B.lo[0] + seg - 1) / seg;
+  const int64_t bh = B.hi[1] - B.lo[1];
+  int64_t rps = dim == 3 ? bh / 20 : (bh * nseg + 3071) / 3072;
   return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
 }
 
@@ -599,8 +784,9 @@ int64_t jacobian_strip_height(const Box &B) {
 }
 
 // number of one-wave groups the velocity march uses for a box
-int64_t velocity_march_blocks(const Box &B, int rows_per_strip) {
-  const int64_t nseg = (B.hi[0] - B.lo[0] + VSEG - 1) / VSEG;
+int64_t velocity_march_blocks(const Box &B, int dim, int w, int rows_per_strip) {
+  const int64_t seg = velocity_segment(dim, w);
+  const int64_t nseg = (B.hi[0] - B.lo[0] + seg - 1) / seg;
   const int64_t bh = B.hi[1] - B.lo[1];
   return nseg * ((bh + rows_per_strip - 1) / rows_per_strip) * (B.hi[2] - B.lo[2]);
 }

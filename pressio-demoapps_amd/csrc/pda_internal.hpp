@@ -226,8 +226,8 @@ int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipS
 // rest_done: the launch also covered the rows outside the box
 int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, double *V,
                         hipStream_t s, bool *rest_done);
-int64_t velocity_march_blocks(const Box &B, int rows_per_strip);
-int64_t velocity_strip_height(const Box &B);
+int64_t velocity_march_blocks(const Box &B, int dim, int w, int rows_per_strip);
+int64_t velocity_strip_height(const Box &B, int dim, int w);
 int64_t jacobian_strip_height(const Box &B);
 int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
                           const double *U, double *V, hipStream_t s);

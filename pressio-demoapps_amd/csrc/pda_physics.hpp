@@ -141,50 +141,93 @@ __device__ __forceinline__ void weno5(const double *q, double &un, double &up) {
 // alpha_k / sum(alpha) = N_k / sum(N) with N_k = d_k * prod_{j != k} D_j and
 // D_k = eps^2 + 2 eps B_k + B_k^2 (the reference's denominator).  Equal to
 // weno5() in exact arithmetic; the results differ by a few ulps.
-__device__ __forceinline__ double weno5_side_1div(double q0, double q1, double q2, double q3,
-                                                  double q4, double d0, double d1, double d2) {
-  constexpr double eps = 1.e-6, c1312 = 13. / 12., c16 = 1. / 6.;
+//
+// D of a side window (far, mid, near), near = the reconstructed cell:
+// B = 13/12 (far - 2 mid + near)^2 + 1/4 (far - 4 mid + 3 near)^2.  The
+// operations are fixed (explicit FMAs, far + near summed first), so that a
+// window gives the same D whichever of its two cells' reconstructions reads
+// it: the low-side and high-side values of a cell share their windows.
+__device__ __forceinline__ double w5_dside(double far, double mid, double near) {
+  constexpr double eps = 1.e-6, c1312 = 13. / 12.;
+  const double t = __fma_rn(-2., mid, far + near);
+  const double s = __fma_rn(-4., mid, __fma_rn(3., near, far));
+  const double B = __fma_rn(c1312 * t, t, (0.25 * s) * s);
+  return __fma_rn(B, B, __fma_rn(2. * eps, B, eps * eps));
+}
+// D of the centred window (lo, mid, hi): B = 13/12 (lo - 2 mid + hi)^2 +
+// 1/4 (lo - hi)^2, the same for the window read in either direction
+__device__ __forceinline__ double w5_dmid(double lo, double mid, double hi) {
+  constexpr double eps = 1.e-6, c1312 = 13. / 12.;
+  const double t = __fma_rn(-2., mid, lo + hi);
+  const double s = lo - hi;
+  const double B = __fma_rn(c1312 * t, t, (0.25 * s) * s);
+  return __fma_rn(B, B, __fma_rn(2. * eps, B, eps * eps));
+}
+// reconstruction at the face of cell q2 towards q3 from the windows' D
+__device__ __forceinline__ double w5_value(double q0, double q1, double q2, double q3, double q4,
+                                           double D0, double D1, double D2) {
+  constexpr double c16 = 1. / 6.;
   const double p0 = (2. * q0 - 7. * q1 + 11. * q2) * c16;
   const double p1 = (-q1 + 5. * q2 + 2. * q3) * c16;
   const double p2 = (2. * q2 + 5. * q3 - q4) * c16;
-  const double t0 = q0 - 2. * q1 + q2, s0 = q0 - 4. * q1 + 3. * q2;
-  const double t1 = q1 - 2. * q2 + q3, s1 = q1 - q3;
-  const double t2 = q2 - 2. * q3 + q4, s2 = 3. * q2 - 4. * q3 + q4;
-  const double B0 = c1312 * t0 * t0 + 0.25 * s0 * s0;
-  const double B1 = c1312 * t1 * t1 + 0.25 * s1 * s1;
-  const double B2 = c1312 * t2 * t2 + 0.25 * s2 * s2;
-  const double D0 = eps * eps + 2. * eps * B0 + B0 * B0;
-  const double D1 = eps * eps + 2. * eps * B1 + B1 * B1;
-  const double D2 = eps * eps + 2. * eps * B2 + B2 * B2;
-  const double n0 = d0 * (D1 * D2), n1 = d1 * (D0 * D2), n2 = d2 * (D0 * D1);
+  const double n0 = 0.1 * (D1 * D2), n1 = 0.6 * (D0 * D2), n2 = 0.3 * (D0 * D1);
   return (n0 * p0 + n1 * p1 + n2 * p2) / (n0 + n1 + n2);
+}
+__device__ __forceinline__ double weno5_side_1div(double q0, double q1, double q2, double q3,
+                                                  double q4) {
+  return w5_value(q0, q1, q2, q3, q4, w5_dside(q0, q1, q2), w5_dmid(q1, q2, q3),
+                  w5_dside(q4, q3, q2));
 }
 
 // q = {q_{i-2}, ..., q_{i+3}}; un, up: left and right states of face i+1/2
 __device__ __forceinline__ void weno5_1div(const double *q, double &un, double &up) {
-  un = weno5_side_1div(q[0], q[1], q[2], q[3], q[4], 0.1, 0.6, 0.3);
+  un = weno5_side_1div(q[0], q[1], q[2], q[3], q[4]);
   // mirrored stencil: the right state of the face is the left-biased
   // reconstruction of the reversed sequence
-  up = weno5_side_1div(q[5], q[4], q[3], q[2], q[1], 0.1, 0.6, 0.3);
+  up = weno5_side_1div(q[5], q[4], q[3], q[2], q[1]);
 }
 
-// WENO3 counterpart of weno5_1div (impl/weno3.hpp:56-132)
+// Both face values of one cell from q = cells i-2 .. i+2: hi = the left
+// state of face i+1/2, lo = the right state of face i-1/2.  Bit-identical to
+// the two weno5_side_1div calls of weno5_1div at those faces; the three
+// windows' D are formed once.
+__device__ __forceinline__ void weno5_cell_1div(double q0, double q1, double q2, double q3,
+                                                double q4, double &hi, double &lo) {
+  const double D0 = w5_dside(q0, q1, q2), D1 = w5_dmid(q1, q2, q3), D2 = w5_dside(q4, q3, q2);
+  hi = w5_value(q0, q1, q2, q3, q4, D0, D1, D2);
+  lo = w5_value(q4, q3, q2, q1, q0, D2, D1, D0);
+}
+
+// WENO3 counterpart of weno5_1div (impl/weno3.hpp:56-132).  D of the
+// two-cell window (a, b): B = (a - b)^2, the same in either direction.
+__device__ __forceinline__ double w3_d(double a, double b) {
+  constexpr double eps = 1.e-6;
+  const double B = (a - b) * (a - b);
+  return __fma_rn(B, B, __fma_rn(2. * eps, B, eps * eps));
+}
+// the face values of cell b (neighbours a below, c above) from the windows'
+// D: hi at the face towards c, lo at the face towards a
+__device__ __forceinline__ double w3_hi(double a, double b, double c, double Dab, double Dbc) {
+  constexpr double c13 = 1. / 3., c23 = 2. / 3.;
+  const double p0 = (-a + 3. * b) * 0.5, p1 = (b + c) * 0.5;
+  const double n0 = c13 * Dbc, n1 = c23 * Dab;
+  return (n0 * p0 + n1 * p1) / (n0 + n1);
+}
+__device__ __forceinline__ double w3_lo(double a, double b, double c, double Dab, double Dbc) {
+  constexpr double c13 = 1. / 3., c23 = 2. / 3.;
+  const double p0 = (a + b) * 0.5, p1 = (3. * b - c) * 0.5;
+  const double n0 = c23 * Dbc, n1 = c13 * Dab;
+  return (n0 * p0 + n1 * p1) / (n0 + n1);
+}
 __device__ __forceinline__ void weno3_1div(const double *q, double &un, double &up) {
-  constexpr double eps = 1.e-6, c13 = 1. / 3., c23 = 2. / 3.;
-  {
-    const double p0 = (-q[0] + 3. * q[1]) * 0.5, p1 = (q[1] + q[2]) * 0.5;
-    const double B0 = (q[0] - q[1]) * (q[0] - q[1]), B1 = (q[1] - q[2]) * (q[1] - q[2]);
-    const double D0 = eps * eps + 2. * eps * B0 + B0 * B0, D1 = eps * eps + 2. * eps * B1 + B1 * B1;
-    const double n0 = c13 * D1, n1 = c23 * D0;
-    un = (n0 * p0 + n1 * p1) / (n0 + n1);
-  }
-  {
-    const double p0 = (q[1] + q[2]) * 0.5, p1 = (3. * q[2] - q[3]) * 0.5;
-    const double B0 = (q[1] - q[2]) * (q[1] - q[2]), B1 = (q[2] - q[3]) * (q[2] - q[3]);
-    const double D0 = eps * eps + 2. * eps * B0 + B0 * B0, D1 = eps * eps + 2. * eps * B1 + B1 * B1;
-    const double n0 = c23 * D1, n1 = c13 * D0;
-    up = (n0 * p0 + n1 * p1) / (n0 + n1);
-  }
+  un = w3_hi(q[0], q[1], q[2], w3_d(q[0], q[1]), w3_d(q[1], q[2]));
+  up = w3_lo(q[1], q[2], q[3], w3_d(q[1], q[2]), w3_d(q[2], q[3]));
+}
+// both face values of cell b, bit-identical to weno3_1div's at its faces
+__device__ __forceinline__ void weno3_cell_1div(double a, double b, double c, double &hi, double &lo) {
+  const double Dab = w3_d(a, b), Dbc = w3_d(b, c);
+  hi = w3_hi(a, b, c, Dab, Dbc);
+  lo = w3_lo(a, b, c, Dab, Dbc);
 }
 
 // one side of impl/weno5.hpp:180-434.  c = linear weights {d0,d1,d2};

@@ -1,5 +1,6 @@
-"""CPU bounds model of the velocity march (vel_march_body in
-csrc/pda_box_kernels.hip), the kernel that once read past the grid.
+"""CPU bounds model of the velocity marches in csrc/pda_box_kernels.hip:
+vel_march_body (3D), the kernel that once read past the grid, and
+vel_cc_body (2D, the cell-centred march with halo lanes).
 
 The model restates the kernel's index arithmetic (workgroup -> plane, strip,
 segment; lane -> column; the strip's row loop; the face stencils and their
@@ -37,6 +38,14 @@ def test_segment_width_matches_kernel_source():
     # the kernel maps lanes past the last cell to column nx - 1 for every
     # read that is not along x
     assert "const int64_t ic = i < nx ? i : nx - 1;" in txt
+    # the 2D march: 64 - 2W owned cells per wave, lanes past the last column
+    # that the segment reads hold that column
+    assert "constexpr int vcc_width() { return 64 - 2 * W; }" in txt
+    assert "const int lc = lane < ncell + 2 * W - 1 ? lane : ncell + 2 * W - 1;" in txt
+    assert "load_cell_mapped2<M>(P, B, U, i, j0 - W + p, win[p]);" in txt
+    assert "load_cell_mapped2<M>(P, B, U, i, j0 - 1 + W, nxt);" in txt
+    assert "if (r + 1 < j1) load_cell_mapped2<M>(P, B, U, i, r + 1 + W, nxt);" in txt
+    assert "for (int64_t r = j0 - 2; r < j1; ++r) {" in txt
 
 
 def _map(c, n, mode):
@@ -111,6 +120,59 @@ def march_accesses(box, W, rps, clamp=True):
     return bad, stores
 
 
+def vcc_width(W):
+    return 64 - 2 * W
+
+
+def cc_accesses(box, W, rps, clamp=True):
+    """All (bad reads, stores) of one launch of the 2D cell-centred march:
+    workgroup -> (strip, segment of 64 - 2W cells); lane l holds column
+    iseg - W + min(l, ncell + 2W - 1) (without the clamp when clamp=False);
+    the strip first loads rows j0 - W .. j0 + W - 2 and row j0 - 1 + W (the
+    next row, loaded one step ahead), then row r + 1 + W at each step
+    r = j0 - 1 .. j1 - 2; both indices are mapped by their axis modes.
+    Owned lanes l in [W, W + ncell) read their own cell for the source terms
+    and store rows j0 .. j1 - 1."""
+    lo, hi, n, mode = box
+    NC = vcc_width(W)
+    bw, bh = hi[0] - lo[0], hi[1] - lo[1]
+    nseg = -(-bw // NC)
+    nstrip = -(-bh // rps)
+    bid = np.arange(nseg * nstrip)
+    seg, strip = bid % nseg, bid // nseg
+    iseg = lo[0] + seg * NC
+    ncell = np.minimum(bw - seg * NC, NC)
+    j0 = lo[1] + strip * rps
+    j1 = np.minimum(j0 + rps, hi[1])
+    lane = np.arange(64)[None, :]
+    lc = np.minimum(lane, (ncell + 2 * W - 1)[:, None]) if clamp else np.broadcast_to(lane, (len(bid), 64))
+    i = (iseg[:, None] - W + lc)[:, :, None]
+    own = ((lane >= W) & (lane < W + ncell[:, None]))[:, :, None]
+    # rows read: the first 2W - 1 and the prefetched one, then one per step
+    it = np.arange(2 * W + rps)
+    first = (j0 - W)[:, None] + it[None, :]                      # it <= 2W - 1
+    step_r = (j0 - 1)[:, None] + (it[None, :] - 2 * W)            # it >= 2W: r
+    row = np.where(it[None, :] < 2 * W, first, step_r + 1 + W)
+    live = np.where(it[None, :] < 2 * W, True, step_r + 1 < j1[:, None])
+    R, L = row[:, None, :], live[:, None, :]
+    shape = np.broadcast_shapes(i.shape, R.shape)
+    I, RR, LIVE = (np.broadcast_to(a, shape) for a in (i, R, L))
+    mi, oki = _map(I, n[0], mode[0])
+    mj, okj = _map(RR, n[1], mode[1])
+    bad = int(np.count_nonzero(LIVE & ~(oki & okj)))
+    # owned cells: reads of the cell itself (sources) and stores
+    rs = np.arange(rps)
+    r = (j0[:, None] + rs[None, :])[:, None, :]
+    body = r < j1[:, None, None]
+    shape = np.broadcast_shapes(i.shape, r.shape)
+    I2, R2 = np.broadcast_to(i, shape), np.broadcast_to(r, shape)
+    st = np.broadcast_to(body, shape) & np.broadcast_to(own, shape)
+    ok = (I2 >= 0) & (I2 < n[0]) & (R2 >= 0) & (R2 < n[1])
+    bad += int(np.count_nonzero(st & ~ok))
+    stores = (R2 * n[0] + I2)[st]
+    return bad, stores
+
+
 def boxes(n, W):
     """Every box the library builds for a grid: the interior box [W, n-W),
     and full-grid axes (periodic or mirror) where the grid is at least
@@ -148,7 +210,7 @@ def test_march_stays_in_grid(grid):
             cells = np.array([(k * n[1] + j) * n[0] + i for k in range(lo[2], hi[2])
                               for j in range(lo[1], hi[1]) for i in range(lo[0], hi[0])])
             for rps in heights:
-                bad, stores = march_accesses(box, W, rps)
+                bad, stores = (march_accesses if grid[2] > 1 else cc_accesses)(box, W, rps)
                 assert bad == 0, (box, W, rps, bad)
                 # every box cell is stored exactly once
                 assert len(stores) == len(cells), (box, W, rps)
@@ -160,7 +222,7 @@ def test_march_stays_in_grid(grid):
 def test_model_flags_the_pre_fix_indexing():
     """Before the fix, lanes past the last cell read y faces and sources at
     column nx: outside the grid whenever the box spans the whole x axis."""
-    for grid in ((70, 131, 1), (64, 12, 1), (10, 9, 8)):
+    for grid in ((70, 131, 1), (64, 12, 1), (10, 9, 8), (24, 40, 9)):
         flagged = 0
         for W in (1, 2, 3):
             for box in boxes(grid, W):
@@ -173,9 +235,10 @@ def test_model_flags_the_pre_fix_indexing():
         assert flagged > 0
 
 
-def velocity_march_blocks(box, rps):
+def velocity_march_blocks(box, W, rps):
     lo, hi, n, mode = box
-    return -(-(hi[0] - lo[0]) // VSEG) * -(-(hi[1] - lo[1]) // rps) * (hi[2] - lo[2])
+    seg = VSEG if n[2] > 1 else vcc_width(W)
+    return -(-(hi[0] - lo[0]) // seg) * -(-(hi[1] - lo[1]) // rps) * (hi[2] - lo[2])
 
 
 def test_batch_instance_table():
@@ -190,7 +253,7 @@ def test_batch_instance_table():
         blk = 0
         for q, s in enumerate(sizes):
             box = ((0, 0, 0), (s, s, 1), (s, s, 1), (2, 2, 0))
-            nb = velocity_march_blocks(box, rps)
+            nb = velocity_march_blocks(box, W, rps)
             inst.append((box, off, blk, nb))
             blk_inst += [q] * nb
             blk += nb
@@ -203,6 +266,21 @@ def test_batch_instance_table():
         for q, (box, o, b0, nb) in enumerate(inst):
             end = o + box[2][0] * box[2][1] * m
             assert end <= total and (q + 1 == len(inst) or inst[q + 1][1] == end)
-            bad, stores = march_accesses(box, W, rps)
+            bad, stores = cc_accesses(box, W, rps)
             assert bad == 0
             assert stores.max() * m + m <= end - o
+
+
+def test_cc_model_flags_unclamped_lanes():
+    """Without the lane clamp, the 2D march's lanes past the columns that a
+    short last segment reads hold columns beyond the mirror or periodic
+    image of the grid: the model flags them."""
+    flagged = 0
+    for grid in ((70, 131, 1), (13, 5, 1), (127, 7, 1)):
+        for W in (1, 2, 3):
+            for box in boxes(grid, W):
+                bad, _ = cc_accesses(box, W, 3, clamp=False)
+                if bad:
+                    flagged += 1
+                assert cc_accesses(box, W, 3)[0] == 0
+    assert flagged > 0
