@@ -498,10 +498,15 @@ template <int W> __device__ __forceinline__ int tile_slot(int a, int o) {
   return a == 0 ? 2 * W + o : (o < 0 ? W + o : 3 * W + o);
 }
 
-// LDS of a tile: stage[NR * PER], vst[NR * M]
+// LDS of a tile: stage[NR * PERP], vst[NR * M].  A row's PER = M (1 + 4W) M
+// staged values are held at a stride PERP = PER rounded up to an odd number
+// of doubles: with PER a multiple of 16 doubles (128 bytes, all LDS banks)
+// the face lanes' writes to one block entry of different rows would all
+// fall in one bank.
 template <class Ph, int W> struct WenoTileLds {
   static constexpr int NR = WenoTile<Ph>::TX * WenoTile<Ph>::TY, M = Ph::M;
-  static constexpr int DOUBLES = NR * M * (1 + 4 * W) * M + NR * M;
+  static constexpr int PER = M * (1 + 4 * W) * M, PERP = PER | 1;
+  static constexpr int DOUBLES = NR * PERP + NR * M;
 };
 
 template <class Ph, int W, bool WV>
@@ -512,9 +517,11 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   constexpr int M = Ph::M, TX = WenoTile<Ph>::TX, TY = WenoTile<Ph>::TY, NR = TX * TY;
   constexpr int LPF = WenoTile<Ph>::LPF, NV = M / LPF;
   constexpr int NB = 1 + 4 * W, RS = NB * M, PER = M * RS, G = 2 * W;
+  constexpr int PERP = WenoTileLds<Ph, W>::PERP;
   constexpr int NXF = (TX + 1) * TY, NYF = TX * (TY + 1);
   static_assert(LPF * (NXF + NYF) <= 64 && NV * LPF == M, "LPF lanes per face");
-  double *vst = stage + NR * PER;
+  static_assert(WenoTileLds<Ph, W>::PER == PER, "stage layout");
+  double *vst = stage + NR * PERP;
   const int lane = threadIdx.x, fl = lane / LPF, hv = lane % LPF; // face, half
   const int j0 = hv * NV;                                           // the lane's first dof
   const int64_t nx = B.n[0];
@@ -607,7 +614,7 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   }
   // the face math and the block products apart (see k_jac_weno_staged)
   wave_sync();
-  double *stL = stage + (rL >= 0 ? rL : 0) * PER, *stR = stage + (rR >= 0 ? rR : 0) * PER;
+  double *stL = stage + (rL >= 0 ? rL : 0) * PERP, *stR = stage + (rR >= 0 ? rR : 0) * PERP;
   // phase 1: as the plus cell's left face, offsets -W..-1 (stencil points
   // 0..W-1); as the minus cell's right face, offsets 1..W (points W..2W-1)
 #pragma unroll
@@ -662,7 +669,7 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
       const double sh = plus ? h : -h;
       const int i = plus ? W : W - 1;
       face_cols<M, NV, G>(sh, Jn, Jp, gm, gp, i, hv != 0, b);
-      double *dst = stage + rr * PER + (edge ? int(plus ? slR[0] : slL[0]) : 2 * W) * M;
+      double *dst = stage + rr * PERP + (edge ? int(plus ? slR[0] : slL[0]) : 2 * W) * M;
       if (ph == 0) put_perm_cols<Ph, RS, false, NV>(dst, a, j0, b);
       else put_perm_cols<Ph, RS, true, NV>(dst, a, j0, b);
       if (WV && hv == 0) {
@@ -678,7 +685,7 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   // the rows' sources, one lane per row
   if (lane < NR && lane % TX < nax && lane / TX < nay) {
     const int64_t r = (y0 + lane / TX) * nx + x0 + lane % TX;
-    double *st = stage + lane * PER;
+    double *st = stage + lane * PERP;
     const int s0 = edge ? int(P.slot[r * P.npos]) : 2 * W;
     double self[M][M], acc[M];
 #pragma unroll
@@ -700,9 +707,10 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   for (int cj = 0; cj < nay; ++cj) {
     const int64_t r0 = (y0 + cj) * nx + x0;
     double *dst = J + P.rowbase[r0];
-    const double *src = stage + cj * TX * PER;
-    const int tot = nax * PER;
-    for (int e = lane; e < tot; e += 64) store_stream(dst + e, src[e]);
+    const double *src = stage + cj * TX * PERP;
+    for (int c = 0; c < nax; ++c)
+#pragma unroll
+      for (int e = lane; e < PER; e += 64) store_stream(dst + c * PER + e, src[c * PERP + e]);
   }
 }
 

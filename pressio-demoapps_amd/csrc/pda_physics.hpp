@@ -12,6 +12,19 @@ namespace phys {
 
 constexpr double kEs = 1.e-30;
 
+// n / d from the hardware reciprocal estimate refined by two Newton steps
+// and one residual correction: within an ulp of the rounded quotient for d
+// and n / d in the normal range (the WENO weight sums, the densities and the
+// Roe factors of the velocity's fluxes), without the IEEE division
+// sequence's scaling for denormal and infinite operands.
+__device__ __forceinline__ double fdiv(double n, double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  r = __fma_rn(r, __fma_rn(-d, r, 1.), r);
+  r = __fma_rn(r, __fma_rn(-d, r, 1.), r);
+  const double q = n * r;
+  return __fma_rn(r, __fma_rn(-d, q, n), q);
+}
+
 // ---------------------------------------------------------------- WENO3
 // impl/weno3.hpp:56-132.  q = {q_{i-1}, q_i, q_{i+1}, q_{i+2}}
 __device__ __forceinline__ void weno3(const double *q, double &un, double &up) {
@@ -171,7 +184,7 @@ __device__ __forceinline__ double w5_value(double q0, double q1, double q2, doub
   const double p1 = (-q1 + 5. * q2 + 2. * q3) * c16;
   const double p2 = (2. * q2 + 5. * q3 - q4) * c16;
   const double n0 = 0.1 * (D1 * D2), n1 = 0.6 * (D0 * D2), n2 = 0.3 * (D0 * D1);
-  return (n0 * p0 + n1 * p1 + n2 * p2) / (n0 + n1 + n2);
+  return fdiv(n0 * p0 + n1 * p1 + n2 * p2, n0 + n1 + n2);
 }
 __device__ __forceinline__ double weno5_side_1div(double q0, double q1, double q2, double q3,
                                                   double q4) {
@@ -211,13 +224,13 @@ __device__ __forceinline__ double w3_hi(double a, double b, double c, double Dab
   constexpr double c13 = 1. / 3., c23 = 2. / 3.;
   const double p0 = (-a + 3. * b) * 0.5, p1 = (b + c) * 0.5;
   const double n0 = c13 * Dbc, n1 = c23 * Dab;
-  return (n0 * p0 + n1 * p1) / (n0 + n1);
+  return fdiv(n0 * p0 + n1 * p1, n0 + n1);
 }
 __device__ __forceinline__ double w3_lo(double a, double b, double c, double Dab, double Dbc) {
   constexpr double c13 = 1. / 3., c23 = 2. / 3.;
   const double p0 = (a + b) * 0.5, p1 = (3. * b - c) * 0.5;
   const double n0 = c23 * Dbc, n1 = c13 * Dab;
-  return (n0 * p0 + n1 * p1) / (n0 + n1);
+  return fdiv(n0 * p0 + n1 * p1, n0 + n1);
 }
 __device__ __forceinline__ void weno3_1div(const double *q, double &un, double &up) {
   un = w3_hi(q[0], q[1], q[2], w3_d(q[0], q[1]), w3_d(q[1], q[2]));
@@ -336,7 +349,7 @@ __device__ __forceinline__ void euler_flux(const double *qL, const double *qR, d
   double velL[DIM], velR[DIM];
   const double rL = qL[0], rR = qR[0];
   // one reciprocal per side (the reference divides per component)
-  const double iL = 1. / (rL + kEs), iR = 1. / (rR + kEs);
+  const double iL = fdiv(1., rL + kEs), iR = fdiv(1., rR + kEs);
   double v2L = 0., v2R = 0.;
   for (int d = 0; d < DIM; ++d) {
     velL[d] = qL[1 + d] * iL;
@@ -357,8 +370,8 @@ __device__ __forceinline__ void euler_flux(const double *qL, const double *qR, d
   }
   FL[M - 1] = rL * unL * HL;
   FR[M - 1] = rR * unR * HR;
-  const double RT = sqrt(rR / rL);
-  const double inv = 1. / (1. + RT);
+  const double RT = sqrt(fdiv(rR, rL));
+  const double inv = fdiv(1., 1. + RT);
   double vr2 = 0.;
   for (int d = 0; d < DIM; ++d) {
     const double v = (velL[d] + RT * velR[d]) * inv;
@@ -503,8 +516,8 @@ __device__ __forceinline__ void euler_fjac(const double *qL, const double *qR, d
 // impl/swe_rusanov_flux_values_function.hpp:54-97
 template <int A>
 __device__ __forceinline__ void swe_flux(const double *qL, const double *qR, double g, double *F) {
-  const double hL = qL[0], uL = qL[1] / (hL + kEs), vL = qL[2] / (hL + kEs);
-  const double hR = qR[0], uR = qR[1] / (hR + kEs), vR = qR[2] / (hR + kEs);
+  const double hL = qL[0], uL = fdiv(qL[1], hL + kEs), vL = fdiv(qL[2], hL + kEs);
+  const double hR = qR[0], uR = fdiv(qR[1], hR + kEs), vR = fdiv(qR[2], hR + kEs);
   const double unL = A == 0 ? uL : vL, unR = A == 0 ? uR : vR;
   const double pL = 0.5 * g * hL * hL, pR = 0.5 * g * hR * hR;
   double FL[3], FR[3];
@@ -512,9 +525,10 @@ __device__ __forceinline__ void swe_flux(const double *qL, const double *qR, dou
   FR[0] = hR * unR; FR[1] = hR * unR * uR + (A == 0 ? pR : 0.); FR[2] = hR * unR * vR + (A == 1 ? pR : 0.);
   const double hm = 0.5 * (hL + hR);
   const double sL = sqrt(hL), sR = sqrt(hR);
-  const double um = (unL * sL + unR * sR) / (sL + sR + kEs);
+  const double um = fdiv(unL * sL + unR * sR, sL + sR + kEs);
   const double c = sqrt(g * hm);
-  const double smax = sqrt(um * um) + sqrt(c * c);
+  // sqrt(x * x) is |x| exactly in IEEE arithmetic (no overflow here)
+  const double smax = fabs(um) + fabs(c);
   for (int i = 0; i < 3; ++i) F[i] = 0.5 * (FL[i] + FR[i] + smax * (qL[i] - qR[i]));
 }
 
