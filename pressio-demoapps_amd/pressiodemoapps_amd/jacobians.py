@@ -1,9 +1,11 @@
-"""Timing of jacobian() on the structured workloads outside the headline:
-WENO Jacobians (the staged kernel) and first-order ones, each on a seeded
-+-1 % perturbation of the problem's initial condition, with the algorithmic
-bytes (8 nnz + 8 m N: values written once, state read once) per call.
+"""Timing of jacobian(), rightHandSideAndJacobian() and rightHandSide() on
+the structured workloads outside the headline: WENO Jacobians (the tiled
+and staged kernels) and first-order ones, each on a seeded +-1 % perturbation
+of the problem's initial condition, with the algorithmic bytes (8 nnz + 8 m
+N: values written once, state read once; velocity 8 m (Nst + N)) per call.
 
     python -m pressiodemoapps_amd.jacobians [--cases euler2d-weno5,burgers-weno5] [--reps 5]
+        [--velocity-strip-height H]   (rows per strip of the velocity march; 0: the grid rule)
 
 Cases:
   euler2d-weno5   2D Euler Riemann IC2, 1024^2, WENO5 (13 4x4 blocks per row)
@@ -40,11 +42,12 @@ CASES = {
 }
 
 
-def measure(name, reps, log):
+def measure(name, reps, log, strip_height=0):
     import torch
     from .sedov3d import _time
     mesh, enum, scheme, ic, params = CASES[name]()
     p = create_problem(mesh, enum, scheme, ic, params)
+    p.set_velocity_strip_height(strip_height)
     U0 = p.initialCondition()
     g = torch.Generator(device="cuda").manual_seed(0)
     U = U0 * (1.0 + 0.01 * (torch.rand(U0.numel(), dtype=torch.float64, device="cuda", generator=g) * 2 - 1))
@@ -52,12 +55,14 @@ def measure(name, reps, log):
     V = p.createRightHandSide()
     tj = _time(lambda: p.jacobian(U, 0.0, J), reps)
     tvj = _time(lambda: p.rightHandSideAndJacobian(U, 0.0, V, J), reps)
+    tv = _time(lambda: p.rightHandSide(U, 0.0, V), reps)
     m, n_rows, nst = p.numDofPerCell(), p.totalDofSampleMesh(), p.totalDofStencilMesh()
     bytes_j = 8 * J.nonZeros() + 8 * nst
     r = {"case": name, "rows": n_rows // m, "jacobian_nnz": J.nonZeros(), "jacobian_us": tj * 1e6,
          "jacobian_GBps": bytes_j / tj / 1e9, "rhs_and_jacobian_us": tvj * 1e6,
          "rhs_and_jacobian_GBps": (bytes_j + 8 * n_rows) / tvj / 1e9,
-         "finite": bool(torch.isfinite(J.values).all().item())}
+         "rhs_us": tv * 1e6, "rhs_GBps": 8 * (nst + n_rows) / tv / 1e9,
+         "finite": bool(torch.isfinite(J.values).all().item() and torch.isfinite(V).all().item())}
     log(f"[jacobians] {json.dumps(r)}")
     return r
 
@@ -66,9 +71,10 @@ def main(argv=None):
     ap = argparse.ArgumentParser(prog="pressiodemoapps_amd.jacobians")
     ap.add_argument("--cases", default="euler2d-weno5,burgers-weno5,swe-weno3")
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--velocity-strip-height", type=int, default=0)
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    print(json.dumps([measure(c, a.reps, log) for c in a.cases.split(",")]))
+    print(json.dumps([measure(c, a.reps, log, a.velocity_strip_height) for c in a.cases.split(",")]))
 
 
 if __name__ == "__main__":
