@@ -62,6 +62,53 @@ def test_velocity_and_jacobian_match_oracle(case, oracle, torch):
         assert cases.rel_err(Jvg, Jg) <= 1e-14, cases.rel_err(Jvg, Jg)
 
 
+# grid widths around the 2D velocity march's segment width (64 - 2W owned
+# cells per wavefront: 58, 60, 62 for W = 3, 2, 1), the narrowest grids a
+# full-grid box allows (2W + 1), mirror ghosts with sign flips (slip walls),
+# zero ghosts (Burgers outflow's minus sides) and periodic wraps
+S = pda.InviscidFluxReconstruction
+SEGMENT_EDGE_CASES = (
+    [(f"riemann-{nx}x9-Weno5", pda.Euler2d.Riemann, S.Weno5, (nx, 9), (0, 1, 0, 1), 7, (), 2, None)
+     for nx in (7, 57, 58, 59, 116, 117)]
+    + [(f"periodic-{nx}x8-Weno5", pda.Euler2d.PeriodicSmooth, S.Weno5, (nx, 8), (-1, 1, -1, 1), 7, True, 0, None)
+       for nx in (7, 58, 59)]
+    + [(f"swe-{nx}x{ny}-{s.name}", pda.Swe2d.SlipWall, s, (nx, ny), (-5, 5, -5, 5), 7, (), 1,
+        {"gravity": 9.8, "coriolis": -3.0})
+       for s, nx, ny in ((S.Weno5, 59, 9), (S.Weno3, 60, 7), (S.Weno3, 61, 7), (S.FirstOrder, 62, 5),
+                         (S.FirstOrder, 63, 5))]
+    + [("burgers-outflow-59x9-Weno5", pda.AdvectionDiffusion2d.BurgersOutflow, S.Weno5, (59, 9), (-1, 1, -1, 1), 7,
+        (), 1, {"diffusion": 5e-3, "pulseX": 0.6, "pulseY": 0.5}),
+       ("burgers-periodic-61x7-Weno3", pda.AdvectionDiffusion2d.BurgersPeriodic, S.Weno3, (61, 7), (-1, 1, -1, 1), 5,
+        True, 1, {"diffusion": 5e-3})])
+
+
+@pytest.mark.parametrize("case", SEGMENT_EDGE_CASES, ids=[c[0] for c in SEGMENT_EDGE_CASES])
+def test_velocity_segment_edges(case, oracle, torch):
+    """velocity() at the widths where the march's last segment is full,
+    one cell short or one cell over, at strip heights 1, 4 and 64 (equal
+    bit for bit) against the oracle."""
+    print(f"[gpu-parity] velocity {case[0]}", flush=True)
+    mesh, prob = cases.make(case)
+    ms = oracle.from_mesh(mesh)
+    pb = cases.oracle_problem(case)
+    U = cases.perturbed_state(prob, 3)
+    Ud = torch.from_numpy(U).cuda()
+    out = []
+    for h in (0, 1, 4, 64):
+        prob.set_velocity_strip_height(h)
+        V = prob.createRightHandSide()
+        prob.rightHandSide(Ud, 0.0, V)
+        torch.cuda.synchronize()
+        out.append(V.cpu().numpy())
+    prob.set_velocity_strip_height(0)
+    Vo = oracle.velocity(ms, pb, U)
+    e = cases.rel_err(out[0], Vo)
+    print(f"[gpu-parity] velocity {case[0]}: {e:.2e}", flush=True)
+    assert e <= TOL, e
+    for V in out[1:]:
+        assert np.array_equal(V, out[0])
+
+
 @pytest.mark.parametrize("case", [c for c in cases.CASES if "riemann-Weno5" in c[0]
                                   or "burgers-periodic-Weno5" in c[0]
                                   or "euler1d-sod-Weno3" in c[0]],
