@@ -216,6 +216,13 @@ int launch_velocity(const pda_problem_s &p, const double *U, double t, double *V
 // box); PDA_E_UNSUPPORTED otherwise
 bool velocity_stage_fits(const pda_problem_s &p);
 int launch_velocity_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s);
+// explicit steps of a 1D problem, up to steps1d_per_launch per launch
+// (Uin != Uout)
+constexpr int STEPS1D_MAX = 8;
+bool steps1d_fits(const pda_problem_s &p, int scheme, bool *periodic);
+int steps1d_per_launch(const pda_problem_s &p, int scheme);
+int launch_steps1d(const pda_problem_s &p, int scheme, const double *Uin, double *Uout, int nsteps, double dt,
+                   bool periodic, hipStream_t s);
 int launch_jacobian(const pda_problem_s &p, const double *U, double t, double *J,
                     double *V, hipStream_t s);
 int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64_t ncols,

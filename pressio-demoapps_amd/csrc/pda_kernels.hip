@@ -35,6 +35,161 @@ __global__ void __launch_bounds__(64) k_velocity_stage(View P, const double *__r
   }
 }
 
+// ----------------------------------------------- 1D steps in one launch
+// A small 1D problem is bound by launch latency: an SSPRK3 step is three
+// velocity launches of a few waves each.  k_steps1d advances several steps
+// in one launch.  Workgroup b owns cells [own b, own b + own) and holds them
+// in LDS with H cells on each side, H = W x the number of stages it runs:
+// each stage's values are valid on a region W cells narrower per side than
+// the stage before, so after the last stage the owned cells are exact.  The
+// region is 128 rows (256 lanes, one wave per SIMD of the workgroup's CU:
+// a stage is one dependent chain of face arithmetic per lane, so a second
+// wave per SIMD would double it).  Two lanes per row evaluate its left and
+// right faces (the generic row code's stencil and ghost rule, the
+// one-division WENO weights of the 2D march, the same flux) and exchange
+// the fluxes; the row's velocity and its stage combination are formed as
+// k_velocity_stage forms them, term by term.  Ghosts: the
+// opposite rule (a row reads position g - d < 0 as cell g + d - 1, g + d
+// >= n as g - d + 1, times the side's sign), or periodic wrap.  The state
+// is read from Uin and written to Uout (workgroups read each other's cells).
+constexpr int S1_ROWS = 128;
+
+struct Steps1dCoef {
+  double c[4][4];  // per stage: coefficients of the stage's terms
+  double cv[4];    // per stage: coefficient of the velocity
+};
+
+template <class Ph, int W, int SCHEME>
+__global__ void __launch_bounds__(2 * S1_ROWS) k_steps1d(View P, int64_t n, int periodic, int own, int H,
+                                                        int nsteps, const double *__restrict__ Uin,
+                                                        double *__restrict__ Uout, Steps1dCoef C) {
+  constexpr int M = Ph::M;
+  extern __shared__ double lds[];
+  const int R = own + 2 * H;
+  const int64_t a0 = int64_t(blockIdx.x) * own, base = a0 - H;
+  // buffers of R x M doubles: 0 the step's state, 1 and 2 stage states,
+  // 3..5 the RK4 stage velocities
+  auto buf = [&](int b) { return lds + int64_t(b) * R * M; };
+  for (int e = threadIdx.x; e < R * M; e += blockDim.x) {
+    const int i = e / M, k = e - i * M;
+    int64_t g = base + i;
+    double v = 0.;
+    if (periodic) {
+      g %= n;
+      if (g < 0) g += n;
+      v = Uin[g * M + k];
+    } else if (g >= 0 && g < n) {
+      v = Uin[g * M + k];
+    }
+    buf(0)[e] = v;
+  }
+  __syncthreads();
+  const int i = int(threadIdx.x) >> 1, f = int(threadIdx.x) & 1;
+  const int64_t g = base + i;
+  const bool row = i < R && (periodic || (g >= 0 && g < n));
+  const bool store = row && f == 0;
+  const double h = P.hinv[0];
+  // velocity of row i from state X (every lane takes part in the exchange)
+  auto vel = [&](const double *X, double (&acc)[M]) {
+    double q[M][2 * W];
+#pragma unroll
+    for (int p = 0; p < 2 * W; ++p) {
+      int64_t src = g + f - W + p;
+      int side = -1;
+      if (!periodic) {
+        if (src < 0) {
+          src = 2 * g - src - 1;
+          side = 0;
+        } else if (src >= n) {
+          src = 2 * g - src + 1;
+          side = 1;
+        }
+      }
+      int64_t li = src - base;
+      li = li < 0 ? 0 : (li >= R ? R - 1 : li);
+#pragma unroll
+      for (int k = 0; k < M; ++k) {
+        const double v = X[li * M + k];
+        q[k][p] = side < 0 ? v : P.gsign[0][side][k] * v;
+      }
+    }
+    double um[M], up[M], F[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      if constexpr (W == 1) {
+        um[k] = q[k][0];
+        up[k] = q[k][1];
+      } else if constexpr (W == 2) {
+        phys::weno3_1div(q[k], um[k], up[k]);
+      } else {
+        phys::weno5_1div(q[k], um[k], up[k]);
+      }
+    }
+    Ph::template flux<0>(P, um, up, F);
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const double o = __shfl_xor(F[k], 1);
+      const double FL = f ? o : F[k], FR = f ? F[k] : o;
+      acc[k] = 0.;
+      acc[k] += h * (FL - FR);
+    }
+  };
+  // out = c0 x0 + ... + cv v, the terms in k_velocity_stage's order
+  auto comb = [&](double *out, int st, int nx, const double *x0, const double *x1, const double *x2,
+                  const double *x3, const double (&acc)[M]) {
+    if (!store) return;
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      const int e = i * M + k;
+      double o = C.c[st][0] * x0[e];
+      if (nx > 1) o += C.c[st][1] * x1[e];
+      if (nx > 2) o += C.c[st][2] * x2[e];
+      if (nx > 3) o += C.c[st][3] * x3[e];
+      o += C.cv[st] * acc[k];
+      out[e] = o;
+    }
+  };
+  auto keep = [&](double *kb, const double (&acc)[M]) {
+    if (!store) return;
+#pragma unroll
+    for (int k = 0; k < M; ++k) kb[i * M + k] = acc[k];
+  };
+  for (int s = 0; s < nsteps; ++s) {
+    double acc[M];
+    if constexpr (SCHEME == PDA_SSPRK3) {
+      vel(buf(0), acc);  // u1 = u + dt f(u)
+      comb(buf(1), 0, 1, buf(0), nullptr, nullptr, nullptr, acc);
+      __syncthreads();
+      vel(buf(1), acc);  // u2 = 1/4 u1 + 3/4 u + 1/4 dt f(u1)
+      comb(buf(2), 1, 2, buf(1), buf(0), nullptr, nullptr, acc);
+      __syncthreads();
+      vel(buf(2), acc);  // u = 1/3 u + 2/3 u2 + 2/3 dt f(u2)
+      comb(buf(0), 2, 2, buf(0), buf(2), nullptr, nullptr, acc);
+      __syncthreads();
+    } else {
+      vel(buf(0), acc);  // k1; y1 = u + dt/2 k1
+      keep(buf(3), acc);
+      comb(buf(1), 0, 1, buf(0), nullptr, nullptr, nullptr, acc);
+      __syncthreads();
+      vel(buf(1), acc);  // k2; y2 = u + dt/2 k2
+      keep(buf(4), acc);
+      comb(buf(2), 1, 1, buf(0), nullptr, nullptr, nullptr, acc);
+      __syncthreads();
+      vel(buf(2), acc);  // k3; y3 = u + dt k3
+      keep(buf(5), acc);
+      comb(buf(1), 2, 1, buf(0), nullptr, nullptr, nullptr, acc);
+      __syncthreads();
+      vel(buf(1), acc);  // u = u + dt/6 k1 + dt/3 k2 + dt/3 k3 + dt/6 k4
+      comb(buf(0), 3, 4, buf(0), buf(3), buf(4), buf(5), acc);
+      __syncthreads();
+    }
+  }
+  if (store && i >= H && i < H + own && g < n) {
+#pragma unroll
+    for (int k = 0; k < M; ++k) Uout[g * M + k] = buf(0)[i * M + k];
+  }
+}
+
 template <class Ph, int W, bool WV>
 __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restrict__ U,
                                                    double *__restrict__ V, double *__restrict__ J) {
@@ -886,6 +1041,100 @@ int launch_velocity_stage(const pda_problem_s &p, const double *U, const RkStage
     case PDA_ADR2D: return vel_stage_dispatch<PhAdr>(v, p.w, U, S, s);
     default: set_error("velocity: unsupported family"); return PDA_E_UNSUPPORTED;
   }
+}
+
+// the fused 1D stepping path: 1D Euler on a natural-order full mesh (row r
+// is cell r, its neighbours r -+ 1..w, wrapped when periodic), SSPRK3 or
+// RK4, with the opposite ghost rule or a periodic grid
+bool steps1d_fits(const pda_problem_s &p, int scheme, bool *periodic_out) {
+  const pda_mesh_s &M = *p.mesh;
+  if (p.family != PDA_EULER1D || M.dim != 1 || (scheme != PDA_SSPRK3 && scheme != PDA_RK4)) return false;
+  if (M.n_sample != M.n_stencil || p.view.nrows != M.n_sample) return false;
+  const int64_t n = M.n_sample;
+  if (n < 2 * p.w + 1 || M.stencil < 2 * p.w + 1) return false;
+  bool periodic = true, open = true;
+  for (int64_t r = 0; r < n; ++r) {
+    if (M.g(r, 0) != r) return false;
+    for (int L = 0; L < p.w; ++L) {
+      const int32_t lft = M.g(r, col_of(1, 0, 0, L)), rgt = M.g(r, col_of(1, 0, 1, L));
+      const int64_t wl = ((r - 1 - L) % n + n) % n, wr = (r + 1 + L) % n;
+      const int64_t ol = r - 1 - L >= 0 ? r - 1 - L : -1, orr = r + 1 + L < n ? r + 1 + L : -1;
+      periodic = periodic && lft == wl && rgt == wr;
+      open = open && lft == ol && rgt == orr;
+    }
+  }
+  *periodic_out = periodic;
+  if (periodic) return true;
+  return open && p.view.ghost_rule == GHOST_OPPOSITE;
+}
+
+// steps per launch: the most that leave at least 32 owned rows of the
+// 128-row region (SSPRK3 WENO5: 5 steps, 38 owned rows)
+int steps1d_per_launch(const pda_problem_s &p, int scheme) {
+  const int per = (scheme == PDA_RK4 ? 4 : 3) * p.w;  // halo rows per side per step
+  const int k = (S1_ROWS - 32) / (2 * per);
+  return k < 1 ? 1 : (k > STEPS1D_MAX ? STEPS1D_MAX : k);
+}
+
+int launch_steps1d(const pda_problem_s &p, int scheme, const double *Uin, double *Uout, int nsteps, double dt,
+                   bool periodic, hipStream_t s) {
+  const int kmax = steps1d_per_launch(p, scheme);
+  if (nsteps < 1 || nsteps > kmax) {
+    set_error("steps1d: steps per launch out of range");
+    return PDA_E_INVALID_ARG;
+  }
+  const int nst = scheme == PDA_RK4 ? 4 : 3, nbuf = scheme == PDA_RK4 ? 6 : 3;
+  const int H = nst * p.w * kmax, own = S1_ROWS - 2 * H;
+  const int64_t n = p.mesh->n_sample;
+  constexpr int M = PhEuler1::M;
+  const unsigned threads = 2 * S1_ROWS;
+  const size_t lds = sizeof(double) * size_t(nbuf) * size_t(S1_ROWS) * M;
+  const dim3 grid{unsigned((n + own - 1) / own), 1, 1};
+  if (own < 1 || lds > 64 * 1024) {
+    set_error("steps1d: region too large");
+    return PDA_E_INVALID_ARG;
+  }
+  // the coefficients exactly as the stage-by-stage step forms them
+  Steps1dCoef C{};
+  if (scheme == PDA_SSPRK3) {
+    const double third = 1. / 3., twoThird = 2. / 3., quarter = 1. / 4., threeQuarter = 3. / 4.;
+    C.c[0][0] = 1.;
+    C.cv[0] = dt;
+    C.c[1][0] = quarter;
+    C.c[1][1] = threeQuarter;
+    C.cv[1] = quarter * dt;
+    C.c[2][0] = third;
+    C.c[2][1] = twoThird;
+    C.cv[2] = twoThird * dt;
+  } else {
+    const double h2 = dt / 2., h6 = dt / 6., h3 = dt / 3.;
+    C.c[0][0] = C.c[1][0] = C.c[2][0] = 1.;
+    C.cv[0] = h2;
+    C.cv[1] = h2;
+    C.cv[2] = dt;
+    C.c[3][0] = 1.;
+    C.c[3][1] = h6;
+    C.c[3][2] = h3;
+    C.c[3][3] = h3;
+    C.cv[3] = h6;
+  }
+  const View &v = p.view;
+  const int per = periodic ? 1 : 0;
+#define PDA_S1(WW, SCH) \
+  hipLaunchKernelGGL((k_steps1d<PhEuler1, WW, SCH>), grid, dim3(threads), lds, s, v, n, per, own, H, nsteps, Uin, \
+                     Uout, C)
+  if (scheme == PDA_SSPRK3) {
+    if (p.w == 1) PDA_S1(1, PDA_SSPRK3);
+    else if (p.w == 2) PDA_S1(2, PDA_SSPRK3);
+    else PDA_S1(3, PDA_SSPRK3);
+  } else {
+    if (p.w == 1) PDA_S1(1, PDA_RK4);
+    else if (p.w == 2) PDA_S1(2, PDA_RK4);
+    else PDA_S1(3, PDA_RK4);
+  }
+#undef PDA_S1
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
 }
 
 int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, double *V,

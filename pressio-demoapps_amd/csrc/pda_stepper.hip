@@ -167,6 +167,7 @@ struct pda_stepper_s {
   hipGraphExec_t exec = nullptr;
   const double *g_U = nullptr;       // arguments the captured step was recorded with
   double g_dt = 0., g_t = 0.;
+  bool fused1d = false, periodic1d = false; // 1D problems: several steps per launch
   ~pda_stepper_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -242,6 +243,7 @@ int pda_stepper_create(pda_problem p, int scheme, pda_stepper *out) {
   st->prob = p;
   st->scheme = scheme;
   st->len = p->mesh->n_stencil * p->m;
+  st->fused1d = steps1d_fits(*p, scheme, &st->periodic1d);
   const size_t bytes = sizeof(double) * size_t(st->len);
   hipError_t e = st->aux.alloc(bytes);
   for (int i = 0; i < 4 && e == hipSuccess; ++i) e = st->k[i].alloc(bytes);
@@ -264,6 +266,23 @@ int pda_stepper_advance(pda_stepper st, double *d_U, double t0, double dt, int64
     return PDA_E_INVALID_ARG;
   }
   hipStream_t s = hipStream_t(stream);
+  if (st->fused1d) {
+    // several steps per launch, the state alternating between U and the
+    // stepper's buffer (a launch's workgroups read cells that other
+    // workgroups of it write); one copy back when the count of launches is
+    // odd.  use_graph does not change this path.
+    const int kmax = steps1d_per_launch(*st->prob, st->scheme);
+    double *a = d_U, *b = st->aux.as<double>();
+    for (int64_t done = 0; done < nsteps;) {
+      const int k = int(std::min<int64_t>(nsteps - done, kmax));
+      const int rc = launch_steps1d(*st->prob, st->scheme, a, b, k, dt, st->periodic1d, s);
+      if (rc) return rc;
+      std::swap(a, b);
+      done += k;
+    }
+    if (a != d_U) return lincomb(d_U, {{1., a}}, st->len, s);
+    return PDA_OK;
+  }
   if (!use_graph) {
     for (int64_t i = 0; i < nsteps; ++i) {
       const int rc = step_once(*st, d_U, t0 + double(i) * dt, dt, s);
