@@ -611,40 +611,50 @@ template <> struct WenoTile<PhEuler2> {
   static constexpr int TX = 4, TY = 3, LPF = 2; // (15 + 16) faces x 2 lanes = 62 lanes
 };
 
-// columns j = j0 + v (v < NV) of a face's block at stencil point i, in the
-// axis frame: b[k][v] = sh (Jn[k][j] gm[v][i] + Jp[k][j] gp[v][i]); j0 is 0
-// or NV, chosen by sel
+// this lane's block columns j0 + v of an axis-frame face Jacobian, rows in
+// the physical frame: out[K][v] = Jx[p_a(K)][j0 + v] (j0 = 0, or NV when
+// sel), chosen once per face so the block products below need no selects
+template <class Ph, int NV>
+__device__ __forceinline__ void lane_cols(const double (&Jx)[Ph::M * Ph::M], int a, bool sel,
+                                          double (&out)[Ph::M][NV]) {
+  constexpr int M = Ph::M;
+#pragma unroll
+  for (int K = 0; K < M; ++K)
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const int r0 = AxisPerm<Ph>::p(K, 0), r1 = AxisPerm<Ph>::p(K, 1);
+      const double x0 = sel ? Jx[r0 * M + NV + v] : Jx[r0 * M + v];
+      const double x1 = sel ? Jx[r1 * M + NV + v] : Jx[r1 * M + v];
+      out[K][v] = a == 1 ? x1 : x0;
+    }
+}
+
+// the face's block at stencil point i, physical rows, the lane's columns:
+// b[K][v] = sh (Jn[K][v] gm[v][i] + Jp[K][v] gp[v][i])
 template <int M, int NV, int G>
-__device__ __forceinline__ void face_cols(double sh, const double (&Jn)[M * M], const double (&Jp)[M * M],
-                                          const double (&gm)[NV][G], const double (&gp)[NV][G], int i, bool sel,
-                                          double (&b)[M][NV]) {
+__device__ __forceinline__ void face_cols_p(double sh, const double (&Jn)[M][NV], const double (&Jp)[M][NV],
+                                            const double (&gm)[NV][G], const double (&gp)[NV][G], int i,
+                                            double (&b)[M][NV]) {
 #pragma clang fp contract(off)
 #pragma unroll
   for (int v = 0; v < NV; ++v)
 #pragma unroll
-    for (int k = 0; k < M; ++k) {
-      const double jn = sel ? Jn[k * M + NV + v] : Jn[k * M + v];
-      const double jp = sel ? Jp[k * M + NV + v] : Jp[k * M + v];
-      b[k][v] = sh * (jn * gm[v][i] + jp * gp[v][i]);
-    }
+    for (int k = 0; k < M; ++k) b[k][v] = sh * (Jn[k][v] * gm[v][i] + Jp[k][v] * gp[v][i]);
 }
 
-// store (or add) block columns j0 + v of the axis frame at dst (row stride
-// RS) in the physical frame: dst[K][p_a(j)] = b[p_a(K)][v]
+// store (or add) physical-row block columns at dst (row stride RS): column
+// j0 + v of the axis frame is column p_a(j0 + v) of the physical frame
 template <class Ph, int RS, bool ADD, int NV>
-__device__ __forceinline__ void put_perm_cols(double *dst, int a, int j0, const double (&b)[Ph::M][NV]) {
+__device__ __forceinline__ void put_cols(double *dst, int a, int j0, const double (&b)[Ph::M][NV]) {
   constexpr int M = Ph::M;
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
     const int j = j0 + v;
-    const int J = a == 0 ? AxisPerm<Ph>::p(j, 0) : (a == 1 ? AxisPerm<Ph>::p(j, 1) : AxisPerm<Ph>::p(j, 2));
+    const int J = a == 0 ? AxisPerm<Ph>::p(j, 0) : AxisPerm<Ph>::p(j, 1);
 #pragma unroll
     for (int K = 0; K < M; ++K) {
-      double x = b[AxisPerm<Ph>::p(K, 0)][v];
-      if constexpr (Ph::DIM > 1) x = a == 1 ? b[AxisPerm<Ph>::p(K, 1)][v] : x;
-      if constexpr (Ph::DIM > 2) x = a == 2 ? b[AxisPerm<Ph>::p(K, 2)][v] : x;
-      if constexpr (ADD) dst[K * RS + J] += x;
-      else dst[K * RS + J] = x;
+      if constexpr (ADD) dst[K * RS + J] += b[K][v];
+      else dst[K * RS + J] = b[K][v];
     }
   }
 }
@@ -767,33 +777,43 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
     Ph::template fjac<0>(P, fm, fp, Jn, Jp);
     if constexpr (WV) Ph::template flux<0>(P, fm, fp, F);
   }
+  // the lane's columns of the two flux Jacobians, rows in the physical frame
+  double JnP[M][NV], JpP[M][NV];
+  lane_cols<Ph, NV>(Jn, a, hv != 0, JnP);
+  lane_cols<Ph, NV>(Jp, a, hv != 0, JpP);
   // the face math and the block products apart (see k_jac_weno_staged)
   wave_sync();
   double *stL = stage + (rL >= 0 ? rL : 0) * PERP, *stR = stage + (rR >= 0 ? rR : 0) * PERP;
   // phase 1: as the plus cell's left face, offsets -W..-1 (stencil points
-  // 0..W-1); as the minus cell's right face, offsets 1..W (points W..2W-1)
+  // 0..W-1); as the minus cell's right face, offsets 1..W (points W..2W-1);
+  // the first neighbours' source term enters entry (p_a(K), j) of the axis
+  // frame, row K of the physical one
 #pragma unroll
   for (int m = 0; m < W; ++m) {
     double b[M][NV];
     if (rR >= 0) {
-      face_cols<M, NV, G>(h, Jn, Jp, gm, gp, m, hv != 0, b);
+      face_cols_p<M, NV, G>(h, JnP, JpP, gm, gp, m, b);
       if (m == W - 1) {
 #pragma unroll
-        for (int k = 0; k < M; ++k)
+        for (int K = 0; K < M; ++K)
 #pragma unroll
-          for (int v = 0; v < NV; ++v) b[k][v] += Sources<Ph>::offdiag(P, false, a, k, j0 + v);
+          for (int v = 0; v < NV; ++v)
+            b[K][v] += Sources<Ph>::offdiag(P, false, a, a == 0 ? AxisPerm<Ph>::p(K, 0) : AxisPerm<Ph>::p(K, 1),
+                                            j0 + v);
       }
-      put_perm_cols<Ph, RS, false, NV>(stR + slotR(m - W) * M, a, j0, b);
+      put_cols<Ph, RS, false, NV>(stR + slotR(m - W) * M, a, j0, b);
     }
     if (rL >= 0) {
-      face_cols<M, NV, G>(-h, Jn, Jp, gm, gp, W + m, hv != 0, b);
+      face_cols_p<M, NV, G>(-h, JnP, JpP, gm, gp, W + m, b);
       if (m == 0) {
 #pragma unroll
-        for (int k = 0; k < M; ++k)
+        for (int K = 0; K < M; ++K)
 #pragma unroll
-          for (int v = 0; v < NV; ++v) b[k][v] += Sources<Ph>::offdiag(P, false, a, k, j0 + v);
+          for (int v = 0; v < NV; ++v)
+            b[K][v] += Sources<Ph>::offdiag(P, false, a, a == 0 ? AxisPerm<Ph>::p(K, 0) : AxisPerm<Ph>::p(K, 1),
+                                            j0 + v);
       }
-      put_perm_cols<Ph, RS, false, NV>(stL + slotL(m + 1) * M, a, j0, b);
+      put_cols<Ph, RS, false, NV>(stL + slotL(m + 1) * M, a, j0, b);
     }
   }
   wave_sync();
@@ -803,12 +823,12 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
   for (int m = 0; m + 1 < W; ++m) {
     double b[M][NV];
     if (rR >= 0) {
-      face_cols<M, NV, G>(h, Jn, Jp, gm, gp, W + 1 + m, hv != 0, b);
-      put_perm_cols<Ph, RS, true, NV>(stR + slotR(1 + m) * M, a, j0, b);
+      face_cols_p<M, NV, G>(h, JnP, JpP, gm, gp, W + 1 + m, b);
+      put_cols<Ph, RS, true, NV>(stR + slotR(1 + m) * M, a, j0, b);
     }
     if (rL >= 0) {
-      face_cols<M, NV, G>(-h, Jn, Jp, gm, gp, m, hv != 0, b);
-      put_perm_cols<Ph, RS, true, NV>(stL + slotL(m - W + 1) * M, a, j0, b);
+      face_cols_p<M, NV, G>(-h, JnP, JpP, gm, gp, m, b);
+      put_cols<Ph, RS, true, NV>(stL + slotL(m - W + 1) * M, a, j0, b);
     }
   }
   // diagonal blocks and velocity: x faces as left faces (store), x faces as
@@ -823,10 +843,10 @@ __device__ __forceinline__ void jac_weno_tiled_body(const View &P, const Box &B,
       double b[M][NV];
       const double sh = plus ? h : -h;
       const int i = plus ? W : W - 1;
-      face_cols<M, NV, G>(sh, Jn, Jp, gm, gp, i, hv != 0, b);
+      face_cols_p<M, NV, G>(sh, JnP, JpP, gm, gp, i, b);
       double *dst = stage + rr * PERP + (edge ? int(plus ? slR[0] : slL[0]) : 2 * W) * M;
-      if (ph == 0) put_perm_cols<Ph, RS, false, NV>(dst, a, j0, b);
-      else put_perm_cols<Ph, RS, true, NV>(dst, a, j0, b);
+      if (ph == 0) put_cols<Ph, RS, false, NV>(dst, a, j0, b);
+      else put_cols<Ph, RS, true, NV>(dst, a, j0, b);
       if (WV && hv == 0) {
 #pragma unroll
         for (int k = 0; k < M; ++k) {
