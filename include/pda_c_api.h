@@ -212,7 +212,10 @@ typedef struct pda_stepper_s *pda_stepper;
 int pda_stepper_create(pda_problem p, int scheme, pda_stepper *out);
 /* nsteps steps of size dt from time t0, in place on d_U.  use_graph != 0
    records one step as a hipGraph (re-recorded when d_U or dt change) and
-   replays it per step on the stream. */
+   replays it per step on the stream.  SSPRK3 and RK4 on a 1D Euler problem
+   over a natural-order full mesh run several steps per launch instead (the
+   stepper's work buffer holds every other launch's state), whatever
+   use_graph says. */
 int pda_stepper_advance(pda_stepper st, double *d_U, double t0, double dt,
                         int64_t nsteps, int use_graph, pda_stream stream);
 void pda_stepper_destroy(pda_stepper st);
