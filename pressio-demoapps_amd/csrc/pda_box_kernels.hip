@@ -256,6 +256,10 @@ __device__ __forceinline__ void vel_cc_body(const View &P, const Box &B, int row
   for (int p = 0; p < NR; ++p) load_cell_mapped2<M>(P, B, U, i, j0 - W + p, win[p]);
   load_cell_mapped2<M>(P, B, U, i, j0 - 1 + W, nxt);
   double qhi[M], fb[M];  // high-side y value of row r; flux of the face below row r
+  // The loop is neither unrolled nor peeled: each face kind has one copy of
+  // its code, so a face's value does not depend on the step that forms it
+  // (the first two steps of a strip form the faces below it).
+#pragma clang loop unroll(disable)
   for (int64_t r = j0 - 2; r < j1; ++r) {
     if (r > j0 - 2) {
 #pragma unroll
