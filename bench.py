@@ -323,10 +323,13 @@ def main():
     # autotune() times back-to-back calls on one state, which stays in cache;
     # in the steps each velocity reads a state that the previous step's
     # 671 MB Jacobian write has evicted, and shares the GPU with a Jacobian,
-    # so the velocity's strip height is also timed in this loop (untimed
-    # set-up: two passes of 3 + 12 steps per candidate, best pass kept).
-    def ctx_us(h):
-        probA.set_velocity_strip_height(h)
+    # so the velocity's strip height is also timed in this loop, jointly
+    # with the Jacobian's (taller Jacobian strips mean fewer Jacobian waves,
+    # which leaves room for velocity waves on the SIMDs); untimed set-up:
+    # two passes of 3 + 12 steps per pair, best pass kept.
+    def ctx_us(hv, hj):
+        probA.set_velocity_strip_height(hv)
+        probB.set_jacobian_strip_height(hj)
         best = 1e300
         for _ in range(2):
             for i in range(3):
@@ -338,12 +341,16 @@ def main():
             torch.cuda.synchronize()
             best = min(best, (time.perf_counter() - t) / 12 * 1e6)
         return best
-    cands = sorted({4, 6, 8, 10, 12, 16, 20, 24, 32, rps})
-    ctx = {h: ctx_us(h) for h in cands}
-    rps = min(ctx, key=ctx.get)
+    vcands = sorted({8, 10, 12, 14, 16, 20, rps})
+    jcands = (0, 4, 6, 8, 10)  # 0: the kernel's own rule (2 rows at 1024^2)
+    ctx = {(hv, hj): ctx_us(hv, hj) for hj in jcands for hv in vcands}
+    rps, jrps = min(ctx, key=ctx.get)
     probA.set_velocity_strip_height(rps)
-    log(f"[bench] strip height in the step loop: " + ", ".join(f"{h}: {ctx[h]:.0f} us" for h in cands)
-        + f" -> {rps}")
+    probB.set_jacobian_strip_height(jrps)
+    for hj in jcands:
+        log(f"[bench] step loop, jacobian strip height {hj or 'rule'}: "
+            + ", ".join(f"{hv}: {ctx[(hv, hj)]:.0f} us" for hv in vcands))
+    log(f"[bench] strip heights in the step loop: velocity {rps}, jacobian {jrps or 'rule'}")
 
     log(f"[bench] warmup {args.warmup}")
     for i in range(args.warmup):
@@ -368,9 +375,12 @@ def main():
     torch.cuda.synchronize()
     wall_serial = time.perf_counter() - t0
 
-    # the roofline kernel in its own pass: jacobian() calls alone, each
-    # between fence-free events on the stream it runs on, each on another
-    # perturbation (the first call, after the synchronize, is not sampled)
+    # the roofline kernel in its own pass, at its own strip-height rule (the
+    # height tuned for the two-stream steps trades the kernel's own speed
+    # for room beside it): jacobian() calls alone, each between fence-free
+    # events on the stream it runs on, each on another perturbation (the
+    # first call, after the synchronize, is not sampled)
+    probB.set_jacobian_strip_height(0)
     s = torch.cuda.current_stream()
     nj = min(args.steps, 20)
     evj = [(KernelEvent(), KernelEvent()) for _ in range(nj)]
@@ -446,6 +456,7 @@ def main():
         "hbm_frac_per_step": (bytes_v + bytes_j) / (wall / args.steps) / 1e9 / HBM_PEAK_GBS,
         "velocity_us": t_v * 1e6,
         "velocity_rows_per_strip": rps,
+        "jacobian_rows_per_strip_in_steps": jrps or "kernel rule",
         "streams": "velocity and jacobian on two streams",
         "serial_ms_per_step": wall_serial / args.steps * 1e3,
         "serial_value": cells / (wall_serial / args.steps),
