@@ -75,6 +75,54 @@ int batch_velocity(const pda_batch_s &b, const double *U, double *V, hipStream_t
                                int64_t(b.h_blk.size()), U, V, s);
 }
 
+// one SSPRK3 or RK4 step with every stage formed in the store of the
+// batched velocity launch (the stage formulas and term order of rk_step,
+// so the states equal rk_step's bit for bit; a stage never writes the
+// vector its velocity reads); other schemes: rk_step
+int batch_step(pda_batch_s &b, int scheme, double *U, double t, double dt, hipStream_t s) {
+  double *k[4] = {b.k[0].as<double>(), b.k[1].as<double>(), b.k[2].as<double>(), b.k[3].as<double>()};
+  double *aux = b.aux.as<double>();
+  const int64_t nb = int64_t(b.h_blk.size());
+  const BatchInst *inst = b.inst.as<BatchInst>();
+  const int32_t *blk = b.blk.as<int32_t>();
+  auto stage = [](double *kout, double *out, std::initializer_list<std::pair<double, const double *>> terms,
+                  double cv) {
+    RkStage S;
+    S.kout = kout;
+    S.out = out;
+    for (auto &x : terms) {
+      S.c[S.nx] = x.first;
+      S.x[S.nx] = x.second;
+      ++S.nx;
+    }
+    S.cv = cv;
+    return S;
+  };
+  auto vs = [&](const double *y, const RkStage &S) {
+    return launch_velocity_batch_stage(b.family, b.w, inst, blk, nb, y, S, s);
+  };
+  if (scheme == PDA_SSPRK3) {
+    const double third = 1. / 3., twoThird = 2. / 3., quarter = 1. / 4., threeQuarter = 3. / 4.;
+    double *aux2 = k[1];
+    int rc = vs(U, stage(nullptr, aux, {{1., U}}, dt));
+    if (!rc) rc = vs(aux, stage(nullptr, aux2, {{quarter, aux}, {threeQuarter, U}}, quarter * dt));
+    if (!rc) rc = vs(aux2, stage(nullptr, U, {{third, U}, {twoThird, aux2}}, twoThird * dt));
+    return rc;
+  }
+  if (scheme == PDA_RK4) {
+    const double h2 = dt / 2., h6 = dt / 6., h3 = dt / 3.;
+    double *aux2 = k[3];
+    int rc = vs(U, stage(k[0], aux, {{1., U}}, h2));
+    if (!rc) rc = vs(aux, stage(k[1], aux2, {{1., U}}, h2));
+    if (!rc) rc = vs(aux2, stage(k[2], aux, {{1., U}}, dt));
+    if (!rc) rc = vs(aux, stage(nullptr, U, {{1., U}, {h6, k[0]}, {h3, k[1]}, {h3, k[2]}}, h6));
+    return rc;
+  }
+  return rk_step(
+      scheme, [&](const double *y, double, double *f) { return batch_velocity(b, y, f, s); }, U, aux, k,
+      b.total, t, dt, s);
+}
+
 void drop_graph(pda_batch_s &b) {
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
@@ -221,13 +269,9 @@ int pda_batch_advance(pda_batch b, int scheme, double *d_U, double t0, double dt
     return PDA_E_INVALID_ARG;
   }
   hipStream_t s = hipStream_t(stream);
-  double *k[4] = {b->k[0].as<double>(), b->k[1].as<double>(), b->k[2].as<double>(), b->k[3].as<double>()};
-  auto vel = [&](hipStream_t st) {
-    return [&b, st](const double *y, double, double *f) { return batch_velocity(*b, y, f, st); };
-  };
   if (steps_per_graph <= 0) {
     for (int64_t i = 0; i < nsteps; ++i) {
-      const int rc = rk_step(scheme, vel(s), d_U, b->aux.as<double>(), k, b->total, t0 + double(i) * dt, dt, s);
+      const int rc = batch_step(*b, scheme, d_U, t0 + double(i) * dt, dt, s);
       if (rc) return rc;
     }
     return PDA_OK;
@@ -237,8 +281,7 @@ int pda_batch_advance(pda_batch b, int scheme, double *d_U, double t0, double dt
     drop_graph(*b);
     PDA_HIP_CHECK(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
     int rc = PDA_OK;
-    for (int i = 0; i < spg && !rc; ++i)
-      rc = rk_step(scheme, vel(b->cap), d_U, b->aux.as<double>(), k, b->total, t0 + double(i) * dt, dt, b->cap);
+    for (int i = 0; i < spg && !rc; ++i) rc = batch_step(*b, scheme, d_U, t0 + double(i) * dt, dt, b->cap);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (rc) {
@@ -259,7 +302,7 @@ int pda_batch_advance(pda_batch b, int scheme, double *d_U, double t0, double dt
   int64_t done = 0;
   for (; done + spg <= nsteps; done += spg) PDA_HIP_CHECK(hipGraphLaunch(b->exec, s));
   for (; done < nsteps; ++done) {
-    const int rc = rk_step(scheme, vel(s), d_U, b->aux.as<double>(), k, b->total, t0 + double(done) * dt, dt, s);
+    const int rc = batch_step(*b, scheme, d_U, t0 + double(done) * dt, dt, s);
     if (rc) return rc;
   }
   return PDA_OK;

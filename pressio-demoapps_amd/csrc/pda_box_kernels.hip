@@ -229,9 +229,15 @@ __device__ __forceinline__ void recon_cell(const double *s, double &hi, double &
   }
 }
 
-template <class Ph, int W>
+// STG: instead of V, each row writes the explicit stage of S (k_i = v into
+// S.kout when set, and S.out = c_0 x_0 + ... + cv v in the order of
+// k_velocity_stage and k_lincomb, so a fused step equals the launches it
+// replaces bit for bit); the stage's vectors are indexed from soff (a
+// batch instance's state offset)
+template <class Ph, int W, bool STG>
 __device__ __forceinline__ void vel_cc_body(const View &P, const Box &B, int rows_per_strip, int64_t bid,
-                                            const double *__restrict__ U, double *__restrict__ V) {
+                                            const double *__restrict__ U, double *__restrict__ V,
+                                            const RkStage &S, int64_t soff) {
   static_assert(Ph::DIM == 2, "2D march");
   constexpr int M = Ph::M, NC = vcc_width<W>(), NR = 2 * W - 1;
   const int lane = threadIdx.x;
@@ -310,8 +316,20 @@ __device__ __forceinline__ void vel_cc_body(const View &P, const Box &B, int row
         if (own) {
           const int64_t c = r * nx + i;
           Sources<Ph>::apply(P, c, false, U, int32_t(c), acc, nullptr);
+          if constexpr (STG) {
 #pragma unroll
-          for (int k = 0; k < M; ++k) V[c * M + k] = acc[k];
+            for (int k = 0; k < M; ++k) {
+              const int64_t e = soff + c * M + k;
+              if (S.kout) S.kout[e] = acc[k];
+              double o = S.c[0] * S.x[0][e];
+              for (int q = 1; q < S.nx; ++q) o += S.c[q] * S.x[q][e];
+              o += S.cv * acc[k];
+              S.out[e] = o;
+            }
+          } else {
+#pragma unroll
+            for (int k = 0; k < M; ++k) V[c * M + k] = acc[k];
+          }
         }
       }
 #pragma unroll
@@ -327,10 +345,11 @@ __device__ __forceinline__ void vel_cc_body(const View &P, const Box &B, int row
 // registers without spilling
 template <class Ph, int W> constexpr int vcc_waves() { return W >= 2 && Ph::M >= 3 ? 3 : 4; }
 
-template <class Ph, int W>
+template <class Ph, int W, bool STG>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(vcc_waves<Ph, W>()))) k_vel_cc(View P, Box B, int rows_per_strip,
-                                                  const double *__restrict__ U, double *__restrict__ V) {
-  vel_cc_body<Ph, W>(P, B, rows_per_strip, xcd_contiguous(int64_t(blockIdx.x), int64_t(gridDim.x)), U, V);
+                                                  const double *__restrict__ U, double *__restrict__ V,
+                                                  RkStage S) {
+  vel_cc_body<Ph, W, STG>(P, B, rows_per_strip, xcd_contiguous(int64_t(blockIdx.x), int64_t(gridDim.x)), U, V, S, 0);
 }
 
 template <class Ph, int W>
@@ -343,13 +362,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(Ph::DIM
 // many independent instances (same family and scheme, each with its own
 // grid, parameters and state slice) in one launch: blk_inst maps a
 // workgroup to its instance, whose workgroups are numbered from blk0
-template <class Ph, int W>
+template <class Ph, int W, bool STG>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(vcc_waves<Ph, W>()))) k_vel_march2d_batch(const BatchInst *__restrict__ inst,
                                                            const int32_t *__restrict__ blk_inst,
                                                            const double *__restrict__ U,
-                                                           double *__restrict__ V) {
+                                                           double *__restrict__ V, RkStage S) {
   const BatchInst &I = inst[blk_inst[blockIdx.x]];
-  vel_cc_body<Ph, W>(I.P, I.B, I.rps, int64_t(blockIdx.x) - I.blk0, U + I.off, V + I.off);
+  // the stage's vectors are batch states: the instance's slices from I.off
+  vel_cc_body<Ph, W, STG>(I.P, I.B, I.rps, int64_t(blockIdx.x) - I.blk0, U + I.off, STG ? nullptr : V + I.off, S,
+                          I.off);
 }
 
 // ------------------------------------------------- first-order Jacobian 2D
@@ -697,9 +718,10 @@ static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream
   if (p.vel_rps > 0) rps = p.vel_rps; // measured by pda_problem_autotune
   const dim3 grid{unsigned(velocity_march_blocks(B, Ph::DIM, p.w, int(rps))), 1, 1};
   if constexpr (Ph::DIM == 2) {
-    if (p.w == 1) hipLaunchKernelGGL((k_vel_cc<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
-    else if (p.w == 2) hipLaunchKernelGGL((k_vel_cc<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
-    else hipLaunchKernelGGL((k_vel_cc<Ph, 3>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
+    const RkStage S{};
+    if (p.w == 1) hipLaunchKernelGGL((k_vel_cc<Ph, 1, false>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V, S);
+    else if (p.w == 2) hipLaunchKernelGGL((k_vel_cc<Ph, 2, false>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V, S);
+    else hipLaunchKernelGGL((k_vel_cc<Ph, 3, false>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V, S);
   } else {
     if (p.w == 1) hipLaunchKernelGGL((k_vel_march<Ph, 1>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
     else if (p.w == 2) hipLaunchKernelGGL((k_vel_march<Ph, 2>), grid, dim3(64), 0, s, p.view, B, int(rps), U, V);
@@ -707,6 +729,30 @@ static int vel_box(const pda_problem_s &p, const double *U, double *V, hipStream
   }
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
+}
+
+// the 2D box march with an explicit stage formed in its store
+template <class Ph>
+static int vel_box_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s) {
+  const Box &B = p.vbox;
+  int64_t rps = velocity_strip_height(B, Ph::DIM, p.w);
+  if (p.vel_rps > 0) rps = p.vel_rps;
+  const dim3 grid{unsigned(velocity_march_blocks(B, Ph::DIM, p.w, int(rps))), 1, 1};
+  if (p.w == 1) hipLaunchKernelGGL((k_vel_cc<Ph, 1, true>), grid, dim3(64), 0, s, p.view, B, int(rps), U, nullptr, S);
+  else if (p.w == 2) hipLaunchKernelGGL((k_vel_cc<Ph, 2, true>), grid, dim3(64), 0, s, p.view, B, int(rps), U, nullptr, S);
+  else hipLaunchKernelGGL((k_vel_cc<Ph, 3, true>), grid, dim3(64), 0, s, p.view, B, int(rps), U, nullptr, S);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
+}
+
+int launch_velocity_box_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s) {
+  if (!p.vbox.valid || p.vbox.cells() == 0) return PDA_OK;
+  switch (p.family) {
+    case PDA_EULER2D: return vel_box_stage<PhEuler2>(p, U, S, s);
+    case PDA_SWE2D: return vel_box_stage<PhSwe>(p, U, S, s);
+    case PDA_BURGERS2D: return vel_box_stage<PhBurgers>(p, U, S, s);
+  }
+  return PDA_E_UNSUPPORTED;
 }
 
 int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s) {
@@ -795,26 +841,41 @@ int64_t velocity_march_blocks(const Box &B, int dim, int w, int rows_per_strip) 
   return nseg * ((bh + rows_per_strip - 1) / rows_per_strip) * (B.hi[2] - B.lo[2]);
 }
 
-template <class Ph>
+template <class Ph, bool STG>
 static void vel_batch(int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks, const double *U,
-                      double *V, hipStream_t s) {
+                      double *V, const RkStage &S, hipStream_t s) {
   const dim3 grid{unsigned(nblocks), 1, 1};
-  if (w == 1) hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 1>), grid, dim3(64), 0, s, inst, blk, U, V);
-  else if (w == 2) hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 2>), grid, dim3(64), 0, s, inst, blk, U, V);
-  else hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 3>), grid, dim3(64), 0, s, inst, blk, U, V);
+  if (w == 1) hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 1, STG>), grid, dim3(64), 0, s, inst, blk, U, V, S);
+  else if (w == 2) hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 2, STG>), grid, dim3(64), 0, s, inst, blk, U, V, S);
+  else hipLaunchKernelGGL((k_vel_march2d_batch<Ph, 3, STG>), grid, dim3(64), 0, s, inst, blk, U, V, S);
 }
 
-int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
-                          const double *U, double *V, hipStream_t s) {
+template <bool STG>
+static int vel_batch_family(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
+                            const double *U, double *V, const RkStage &S, hipStream_t s) {
   if (nblocks <= 0) return PDA_OK;
   switch (family) {
-    case PDA_EULER2D: vel_batch<PhEuler2>(w, inst, blk, nblocks, U, V, s); break;
-    case PDA_SWE2D: vel_batch<PhSwe>(w, inst, blk, nblocks, U, V, s); break;
-    case PDA_BURGERS2D: vel_batch<PhBurgers>(w, inst, blk, nblocks, U, V, s); break;
+    case PDA_EULER2D: vel_batch<PhEuler2, STG>(w, inst, blk, nblocks, U, V, S, s); break;
+    case PDA_SWE2D: vel_batch<PhSwe, STG>(w, inst, blk, nblocks, U, V, S, s); break;
+    case PDA_BURGERS2D: vel_batch<PhBurgers, STG>(w, inst, blk, nblocks, U, V, S, s); break;
     default: set_error("batch velocity: unsupported family"); return PDA_E_UNSUPPORTED;
   }
   PDA_HIP_CHECK(hipGetLastError());
   return PDA_OK;
+}
+
+int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
+                          const double *U, double *V, hipStream_t s) {
+  return vel_batch_family<false>(family, w, inst, blk, nblocks, U, V, RkStage{}, s);
+}
+
+int launch_velocity_batch_stage(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
+                                const double *U, const RkStage &S, hipStream_t s) {
+  if (!S.out || S.nx < 1 || S.nx > 4) {
+    set_error("batch velocity stage: invalid stage");
+    return PDA_E_INVALID_ARG;
+  }
+  return vel_batch_family<true>(family, w, inst, blk, nblocks, U, nullptr, S, s);
 }
 
 }  // namespace pda

@@ -230,6 +230,8 @@ int launch_apply(const pda_problem_s &p, const double *J, const double *B, int64
 // structured-box kernels; return PDA_E_UNSUPPORTED when the problem has no
 // box kernel (the caller then covers all rows with the generic kernels)
 int launch_velocity_box(const pda_problem_s &p, const double *U, double *V, hipStream_t s);
+// the 2D box march forming an explicit stage in its store
+int launch_velocity_box_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s);
 // rest_done: the launch also covered the rows outside the box
 int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, double *V,
                         hipStream_t s, bool *rest_done);
@@ -238,6 +240,8 @@ int64_t velocity_strip_height(const Box &B, int dim, int w);
 int64_t jacobian_strip_height(const Box &B);
 int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
                           const double *U, double *V, hipStream_t s);
+int launch_velocity_batch_stage(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
+                                const double *U, const RkStage &S, hipStream_t s);
 // view of the generic kernels: every row (rows == nullptr) or a row list
 View generic_view(const pda_problem_s &p, const pda::DevBuf *rows, int64_t count);
 bool has_box_kernels(const pda_problem_s &p);

@@ -1043,13 +1043,31 @@ int launch_velocity(const pda_problem_s &p, const double *U, double, double *V, 
   return launch_velocity_box(p, U, V, s);
 }
 
-bool velocity_stage_fits(const pda_problem_s &p) { return p.view.nrows > 0 && !box_velocity(p); }
+// the problems whose velocity launches can form an explicit stage in their
+// stores: the generic row kernel, and the 2D box march (with the generic
+// rows outside its box)
+bool velocity_stage_fits(const pda_problem_s &p) {
+  return p.view.nrows > 0 && (!box_velocity(p) || p.dim == 2);
+}
 
 int launch_velocity_stage(const pda_problem_s &p, const double *U, const RkStage &S, hipStream_t s) {
   if (!velocity_stage_fits(p)) return PDA_E_UNSUPPORTED;
   if (!S.out || S.nx < 1 || S.nx > 4) {
     set_error("velocity stage: invalid stage");
     return PDA_E_INVALID_ARG;
+  }
+  if (box_velocity(p)) {
+    if (!p.h_vrem_rows.empty()) {
+      const View v = generic_view(p, &p.d_vrem_rows, int64_t(p.h_vrem_rows.size()));
+      int rc = PDA_E_UNSUPPORTED;
+      switch (p.family) {
+        case PDA_EULER2D: rc = vel_stage_dispatch<PhEuler2>(v, p.w, U, S, s); break;
+        case PDA_SWE2D: rc = vel_stage_dispatch<PhSwe>(v, p.w, U, S, s); break;
+        case PDA_BURGERS2D: rc = vel_stage_dispatch<PhBurgers>(v, p.w, U, S, s); break;
+      }
+      if (rc) return rc;
+    }
+    return launch_velocity_box_stage(p, U, S, s);
   }
   const View v = generic_view(p, nullptr, 0);
   switch (p.family) {
