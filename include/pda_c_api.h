@@ -156,11 +156,12 @@ int pda_problem_autotune(pda_problem p, const double *d_U, pda_stream stream,
 int pda_problem_set_velocity_strip_height(pda_problem p, int rows_per_strip);
 
 /* Sets the strip height (grid rows marched by one workgroup) of the
-   first-order 2D Euler Jacobian kernel used by jacobian() without V on a
-   structured mesh (no counterpart in the reference): 0 restores the
-   grid-size rule (about 16k workgroups), 1..64 fixes the height; other
-   values are PDA_E_INVALID_ARG.  Problems that do not use that kernel
-   accept and ignore it.  The Jacobian values do not depend on the height. */
+   first-order 2D and 3D Euler Jacobian kernels used by jacobian() without V
+   on a structured mesh (no counterpart in the reference): 0 restores the
+   grid-size rule (about 16k workgroups in 2D, 64k in 3D), 1..64 fixes the
+   height; other values are PDA_E_INVALID_ARG.  Problems that do not use
+   those kernels accept and ignore it.  The Jacobian values do not depend on
+   the height. */
 int pda_problem_set_jacobian_strip_height(pda_problem p, int rows_per_strip);
 
 /* rightHandSideAndJacobian(U, t, V, J) (adapter_cpp.hpp:201-208) when d_V is

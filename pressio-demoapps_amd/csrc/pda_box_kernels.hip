@@ -10,7 +10,9 @@
 //     from rows held in registers;
 //   - first-order Jacobian: the blocks are assembled in LDS in CSR order and
 //     leave as one contiguous run per row segment (coalesced 16-byte stores);
-//     for 2D Euler two lanes per cell evaluate each face once.
+//     for 2D Euler two lanes per cell evaluate each face once, for 3D Euler
+//     three lanes per cell (one per axis) march the x lines of a strip of
+//     planes.
 // Cell ids follow the structured layout gid = j*nx + i (graph row == gid).
 #include "pda_rows.hpp"
 
@@ -701,6 +703,177 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler2d(View P, Box B, int rows_p
   jac_fo_euler2d_seg(P, B, rows_per_strip, int64_t(blockIdx.x) - nframe, U, J);
 }
 
+// ------------------------- first-order Euler 3D Jacobian, marching x lines
+// A wavefront owns a segment of up to E consecutive box cells of an x line
+// (j, k) and marches in z over a strip of planes; consecutive workgroups take
+// consecutive lines j of the same strip, so the waves in flight write
+// neighbouring ranges of the CSR values.  On each line lanes 0..E
+// ("x lanes") evaluate the x faces of the segment (lane l: the face on the
+// left of cell l, lane ncell the face on the right of the last cell), lanes
+// E+1..2E ("y lanes") the two y faces of cell lane-E-1 and lanes 2E+1..3E
+// ("z lanes") the two z faces of cell lane-2E-1; lanes past 3E repeat the
+// last z lane and write nothing.  The 3D Euler flux along axis a is the flux
+// along x with the momentum components 1 and 1+a exchanged, so every lane
+// runs the x-face code on exchanged states and writes entry (r, c) of its
+// blocks to row p(r), column p(c), p that exchange.  For a cell and an axis
+// with faces lo (below) and up (above):
+//   block -a = h JL(lo),  block +a = -h JR(up),  diagonal += h (JR(lo) - JL(up)).
+// An x lane forms, from its one face, the -x block of the cell on its right
+// and the +x block of the cell on its left, and takes JL of the next face
+// from the next lane; a y or z lane evaluates both faces of its cell.  The
+// diagonal adds the axes in LDS in the order x, y, z, as the generic row
+// code does.  The line's blocks are assembled in LDS in CSR order (per
+// cell: scalar row r, then the blocks [-z, -y, -x, self, +x, +y, +z], then
+// column) and leave as one contiguous range of 16-byte streaming stores
+// that the next line does not wait for; the next line's states are loaded
+// while the current line's blocks are formed.  The box keeps every face's
+// two cells inside the grid (no ghost, no BC factor), and its cells' CSR
+// rows hold the seven blocks in that order, consecutive cells consecutive.
+// (Running the rows outside the box in the same launch, as the 2D kernel
+// does, measured slower here: 6.24 vs 5.09 ms per jacobian() at 256^3, the
+// 390k boundary rows' workgroups holding this kernel's registers and LDS.)
+template <int E>
+__global__ void __launch_bounds__(64) k_jac_fo_euler3d(View P, Box B, int rows_per_strip,
+                                                        const double *__restrict__ U,
+                                                        double *__restrict__ J) {
+  using Ph = PhEuler3;
+  using Side = Ph::Side;
+  constexpr int M = 5, RS = 35, PER = M * RS;
+  static_assert(3 * E + 1 <= 64, "one x lane per face, one y and one z lane per cell");
+  __shared__ double stage[E * PER];
+  const int lane = threadIdx.x;
+  const int ax = lane <= E ? 0 : (lane <= 2 * E ? 1 : 2);
+  const int l = ax == 0 ? lane : (ax == 1 ? lane - E - 1 : lane - 2 * E - 1);
+  const bool xl = ax == 0;
+  const int64_t nx = B.n[0], nxy = B.n[0] * B.n[1];
+  const int64_t bw = B.hi[0] - B.lo[0], bh = B.hi[1] - B.lo[1];
+  const int64_t nseg = (bw + E - 1) / E;
+  const int64_t bid = blockIdx.x;
+  const int64_t seg = bid % nseg, jj = B.lo[1] + (bid / nseg) % bh;
+  const int64_t strip = bid / (nseg * bh);
+  const int64_t iseg = B.lo[0] + seg * E;
+  const int ncell = int(bw - seg * E < E ? bw - seg * E : E);
+  const int64_t k0 = B.lo[2] + strip * rows_per_strip;
+  const int64_t k1 = k0 + rows_per_strip < B.hi[2] ? k0 + rows_per_strip : B.hi[2];
+  // the lane's cell (x lanes: the cell on the right of the face) and its
+  // states: x lanes read (cell-1, cell), y and z lanes (cell-s, cell, cell+s)
+  const int ci = xl ? (l <= ncell ? l : ncell) : (l < ncell ? l : ncell - 1);
+  const int64_t st = ax == 1 ? nx : nxy;
+  const int64_t sa = xl ? 1 : st, sc = xl ? 0 : st;
+  const bool own = l < ncell;
+  const double h = P.hinv[ax];
+  // row r of an entry goes to row p(r) of the stage, column c to p(c)
+  auto pr = [&](int r) { return r == 1 ? 1 + ax : (r == 1 + ax ? 1 : r); };
+  // blocks from the lane's faces: block 1 is the -a block of the lane's
+  // cell, block 2 the +a block of the cell on the left (x lanes) or of the
+  // lane's cell (y and z lanes)
+  const bool v1 = own, v2 = xl ? (l >= 1 && l <= ncell) : own;
+  const int c2 = xl ? l - 1 : l;
+  double *d1 = stage + (v1 ? l : 0) * PER + (2 - ax) * M;
+  double *d2 = stage + (v2 ? c2 : 0) * PER + (4 + ax) * M;
+  double *dd = stage + (own ? l : 0) * PER + 3 * M;
+  double qa[M], qb[M], qc[M];
+  {
+    const int64_t c = k0 * nxy + jj * nx + iseg + ci;
+    load_cell<M>(U, c - sa, qa);
+    load_cell<M>(U, c, qb);
+    load_cell<M>(U, c + sc, qc);
+  }
+  // the first line's states are consumed here, so the loop head waits only
+  // for the states prefetched in the previous iteration
+#pragma unroll
+  for (int k = 0; k < M; ++k) asm volatile("" : : "v"(qa[k]), "v"(qb[k]), "v"(qc[k]));
+#pragma clang loop unroll(disable)
+  for (int64_t kk = k0; kk < k1; ++kk) {
+    double ea[M], eb[M], ec[M];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      ea[k] = qa[pr(k)];
+      eb[k] = qb[pr(k)];
+      ec[k] = qc[pr(k)];
+    }
+    Side A0, A1, B0, B1;
+    Ph::sides<0>(P, ea, eb, A0, A1);
+    // x lanes evaluate a second face from equal states and discard it
+    Ph::sides<0>(P, eb, ec, B0, B1);
+    {
+      // prefetch the next line (the last line reloads itself, so the loads
+      // and their wait counts are static) and read the line's first CSR
+      // offset, both arriving while the blocks are formed
+      const int64_t kn = kk + 1 < k1 ? kk + 1 : kk;
+      const int64_t c = kn * nxy + jj * nx + iseg + ci;
+      load_cell<M>(U, c - sa, qa);
+      load_cell<M>(U, c, qb);
+      load_cell<M>(U, c + sc, qc);
+    }
+    const int cell0 = __builtin_amdgcn_readfirstlane(int(kk * nxy + jj * nx + iseg));
+    const int64_t rb0 = P.rowbase[cell0];
+    Side nA0;
+    {
+      const double *pa = reinterpret_cast<const double *>(&A0);
+      double *pn = reinterpret_cast<double *>(&nA0);
+#pragma unroll
+      for (int q = 0; q < int(sizeof(Side) / sizeof(double)); ++q) pn[q] = lane_next(pa[q]);
+    }
+    const Side Up = pick(xl, nA0, B0);  // JL of the face above the cell
+    const Side X2 = pick(xl, A1, B1);
+    if (v1) {
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) d1[pr(r) * RS + pr(c)] = h * Ph::entry<0>(P, A0, r, c);
+    }
+    if (v2) {
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) d2[pr(r) * RS + pr(c)] = -h * Ph::entry<0>(P, X2, r, c);
+    }
+    double part[M * M];
+#pragma unroll
+    for (int r = 0; r < M; ++r)
+#pragma unroll
+      for (int c = 0; c < M; ++c) part[r * M + c] = (Ph::entry<0>(P, A1, r, c) - Ph::entry<0>(P, Up, r, c)) * h;
+    if (own && xl) {
+#pragma unroll
+      for (int r = 0; r < M; ++r)
+#pragma unroll
+        for (int c = 0; c < M; ++c) dd[r * RS + c] = part[r * M + c];
+    }
+#pragma unroll
+    for (int phase = 1; phase < 3; ++phase) {
+      wave_sync();
+      if (own && ax == phase) {
+#pragma unroll
+        for (int r = 0; r < M; ++r)
+#pragma unroll
+          for (int c = 0; c < M; ++c) dd[pr(r) * RS + pr(c)] += part[r * M + c];
+      }
+    }
+    wave_sync();
+    // copy the line's values out: 16-byte chunks after a leading value that
+    // is not 16-byte aligned; a fixed, unrolled count per lane (lanes past
+    // the end store the last chunk again, with the same value) lets the
+    // compiler count the stores, so the next line does not wait for them
+    const int head = int(rb0 & 1);
+    const int total = ncell * PER;
+    const int npairs = (total - head) >> 1;
+    double2 *dst = reinterpret_cast<double2 *>(J + rb0 + head);
+    int lq = lane;
+    asm volatile("" : "+v"(lq));
+#pragma unroll
+    for (int it = 0; it < (E * PER / 2 + 63) / 64; ++it) {
+      const int q0 = lq + 64 * it;
+      const int q = q0 < npairs ? q0 : npairs - 1;
+      const double *src = stage + head + 2 * q;
+      store_stream(dst + q, make_double2(src[0], src[1]));
+    }
+    if (lane == 0 && head) store_stream(J + rb0, stage[0]);
+    if (lane == 1 && ((total - head) & 1)) store_stream(J + rb0 + total - 1, stage[total - 1]);
+    wave_sync();
+  }
+}
+
 }  // namespace
 
 bool has_box_kernels(const pda_problem_s &p) {
@@ -831,6 +1004,31 @@ int64_t jacobian_strip_height(const Box &B) {
   const int64_t nseg = (B.hi[0] - B.lo[0] + ESEG - 1) / ESEG;
   const int64_t rps = ((B.hi[1] - B.lo[1]) * nseg) / 16384;
   return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
+}
+
+// cells per x segment of the first-order 3D Euler Jacobian march
+constexpr int J3SEG = 16;
+
+// default strip height (planes per workgroup) of that march, the fastest
+// of 1..64 at 256^3 (5.09 ms per jacobian() at 4, 5.29 at 2, 5.60 at 8)
+constexpr int64_t J3_STRIP = 4;
+
+int launch_jacobian_fo_euler3d_box(const pda_problem_s &p, const double *U, double *J, hipStream_t s) {
+  const Box &B = p.box;
+  for (int a = 0; a < 3; ++a)
+    if (B.mode[a] != 0 || B.lo[a] < 1 || B.hi[a] > B.n[a] - 1) {
+      set_error("first-order 3D Jacobian march: box stencil leaves the grid");
+      return PDA_E_INVALID_ARG;
+    }
+  int64_t rps = J3_STRIP;
+  if (p.jac_rps > 0) rps = p.jac_rps; // pda_problem_set_jacobian_strip_height
+  const int64_t nseg = (B.hi[0] - B.lo[0] + J3SEG - 1) / J3SEG;
+  const int64_t nstrip = (B.hi[2] - B.lo[2] + rps - 1) / rps;
+  const int64_t nb = nseg * (B.hi[1] - B.lo[1]) * nstrip;
+  hipLaunchKernelGGL((k_jac_fo_euler3d<J3SEG>), dim3(unsigned(nb)), dim3(64), 0, s, generic_view(p, nullptr, 0),
+                     B, int(rps), U, J);
+  PDA_HIP_CHECK(hipGetLastError());
+  return PDA_OK;
 }
 
 // number of one-wave groups the velocity march uses for a box

@@ -199,7 +199,7 @@ struct pda_problem_s {
   pda::DevBuf d_vrem_rows;
   int use_box = 1;                   // 0 forces the generic kernels everywhere
   int vel_rps = 0;                   // velocity march strip height (0: from the grid size)
-  int jac_rps = 0;                   // first-order 2D Euler Jacobian strip height (0: from the grid size)
+  int jac_rps = 0;                   // first-order 2D/3D Euler Jacobian strip height (0: from the grid size)
   pda::View view;
   pda::DevBuf d_graph, d_ghost_row, d_near_rows, d_ghost, d_bcf;
   pda::DevBuf d_rowptr, d_colidx, d_rowbase, d_nblk, d_slot, d_bcol;
@@ -238,6 +238,9 @@ int launch_jacobian_box(const pda_problem_s &p, const double *U, double *J, doub
 int64_t velocity_march_blocks(const Box &B, int dim, int w, int rows_per_strip);
 int64_t velocity_strip_height(const Box &B, int dim, int w);
 int64_t jacobian_strip_height(const Box &B);
+// first-order 3D Euler Jacobian over the cells of p.box of a structured
+// full mesh; the rows outside the box are left to the caller
+int launch_jacobian_fo_euler3d_box(const pda_problem_s &p, const double *U, double *J, hipStream_t s);
 int launch_velocity_batch(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,
                           const double *U, double *V, hipStream_t s);
 int launch_velocity_batch_stage(int family, int w, const BatchInst *inst, const int32_t *blk, int64_t nblocks,

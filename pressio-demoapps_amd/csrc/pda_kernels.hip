@@ -209,20 +209,14 @@ __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restric
 // its contiguous CSR values with coalesced streaming stores (one thread per
 // row would scatter every store over 64 rows); 16 rows consecutive in the
 // CSR leave as one range of 16-byte stores.  Near-boundary rows are left to
-// the generic kernel.
+// the generic kernel.  Used where no structured box kernel applies (sample
+// meshes, 2D meshes given as a graph).
 constexpr int SJ_LANES = 4, SJ_ROWS = 64 / SJ_LANES;
 
 __device__ __forceinline__ int exch(int k, int a) { return k == 1 ? 1 + a : (k == 1 + a ? 1 : k); }
 
-//
-// With BOX the rows are the cells of a structured box of a natural-order
-// full mesh whose first-order stencil stays in the grid (no ghost, no BC
-// factor): row r = (k ny + j) nx + i, neighbours r -+ stride_a, and the
-// sorted CSR columns put the block of r - stride_a at slot D-1-a, the
-// diagonal at D and r + stride_a at D+1+a, so a row needs no graph, ghost or
-// slot lookups before its state loads.
-template <int D, bool BOX>
-__global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, Box B, const double *__restrict__ U,
+template <int D>
+__global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double *__restrict__ U,
                                                               double *__restrict__ J) {
   using Ph = PhEuler<D>;
   constexpr int M = D + 2, NB = 1 + 2 * D, RS = NB * M, PER = M * RS;
@@ -231,13 +225,7 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, Box B, const
   const int lane = threadIdx.x, g = lane / SJ_LANES, a = lane % SJ_LANES;
   const int64_t t = int64_t(blockIdx.x) * SJ_ROWS + g;
   int64_t r = -1;
-  if constexpr (BOX) {
-    const int64_t bw = B.hi[0] - B.lo[0], bh = B.hi[1] - B.lo[1];
-    if (t < bw * bh * (B.hi[2] - B.lo[2])) {
-      const int64_t i = B.lo[0] + t % bw, j = B.lo[1] + (t / bw) % bh, k = B.lo[2] + t / (bw * bh);
-      r = (k * B.n[1] + j) * B.n[0] + i;
-    }
-  } else if (t < P.nvisit) {
+  if (t < P.nvisit) {
     const int64_t rr = P.rows ? P.rows[t] : t;
     if (P.ghost_row[rr] < 0) r = rr;
   }
@@ -249,15 +237,7 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, Box B, const
   double h = 0.;
   if (act) {
     int64_t cells[3];
-    if constexpr (BOX) {
-      const int64_t stride = a == 0 ? 1 : (a == 1 ? B.n[0] : B.n[0] * B.n[1]);
-      cells[0] = r - stride;
-      cells[1] = r;
-      cells[2] = r + stride;
-      sL = D - 1 - a;
-      sR = D + 1 + a;
-      s0 = D;
-    } else {
+    {
       const int32_t *gr = P.graph + r * P.ncols;
       const uint8_t *sl = P.slot + r * P.npos;
       const int cl = colof<D>(a, 0, 0), cr = colof<D>(a, 1, 0);
@@ -1244,20 +1224,17 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
       // first-order Euler Jacobian alone: staged inner rows, then the
       // near-boundary rows with the generic row code
       const View v = generic_view(p, nullptr, 0);
-      const Box nobox;
       if (p.family == PDA_EULER3D && p.use_box && p.box.valid && p.box.cells() > 0) {
-        // the box's rows from their coordinates, the rest with the generic
-        // row code
-        const unsigned nb = unsigned((p.box.cells() + SJ_ROWS - 1) / SJ_ROWS);
-        hipLaunchKernelGGL((k_jac_fo_euler_staged<3, true>), dim3(nb), dim3(64), 0, s, v, p.box, U, J);
-        PDA_HIP_CHECK(hipGetLastError());
-        if (p.h_rem_rows.empty()) return PDA_OK;
+        // the box's x lines marched in z, then the rows outside the box
+        // with the generic row code
+        const int rc = launch_jacobian_fo_euler3d_box(p, U, J, s);
+        if (rc || p.h_rem_rows.empty()) return rc;
         return jac_generic(generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size())), p, U, V, J, s);
       }
       const unsigned nb = unsigned((v.nvisit + SJ_ROWS - 1) / SJ_ROWS);
       if (p.family == PDA_EULER2D)
-        hipLaunchKernelGGL((k_jac_fo_euler_staged<2, false>), dim3(nb), dim3(64), 0, s, v, nobox, U, J);
-      else hipLaunchKernelGGL((k_jac_fo_euler_staged<3, false>), dim3(nb), dim3(64), 0, s, v, nobox, U, J);
+        hipLaunchKernelGGL((k_jac_fo_euler_staged<2>), dim3(nb), dim3(64), 0, s, v, U, J);
+      else hipLaunchKernelGGL((k_jac_fo_euler_staged<3>), dim3(nb), dim3(64), 0, s, v, U, J);
       PDA_HIP_CHECK(hipGetLastError());
       if (p.n_near == 0) return PDA_OK;
       return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);

@@ -484,9 +484,9 @@ class Problem:
         _check(lib().pda_problem_set_velocity_strip_height(self._h, int(rows_per_strip)))
 
     def set_jacobian_strip_height(self, rows_per_strip):
-        """Fix the strip height of the first-order 2D Euler Jacobian kernel
-        (pda_problem_set_jacobian_strip_height); 0 restores the grid-size
-        rule.  The Jacobian values do not depend on the height."""
+        """Fix the strip height of the first-order 2D and 3D Euler Jacobian
+        kernels (pda_problem_set_jacobian_strip_height); 0 restores the
+        grid-size rule.  The Jacobian values do not depend on the height."""
         _check(lib().pda_problem_set_jacobian_strip_height(self._h, int(rows_per_strip)))
 
     rhs = rightHandSide

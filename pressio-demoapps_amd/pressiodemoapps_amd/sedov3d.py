@@ -33,7 +33,7 @@ def _time(fn, reps):
     return ev[0].elapsed_time(ev[1]) / reps * 1e-3
 
 
-def measure(mesh, label, reps, log):
+def measure(mesh, label, reps, log, jheights=()):
     import torch
     pa = create_problem(mesh, Euler3d.SedovSymmetry, S.Weno5, 0, dict(IC))
     pb = create_problem(mesh, Euler3d.SedovSymmetry, S.FirstOrder, 0, dict(IC))
@@ -54,6 +54,14 @@ def measure(mesh, label, reps, log):
          "velocity_us": tv * 1e6, "velocity_rows_per_s": rows / tv, "velocity_GBps": bytes_v / tv / 1e9,
          "jacobian_us": tj * 1e6, "jacobian_rows_per_s": rows / tj, "jacobian_GBps": bytes_j / tj / 1e9,
          "finite": bool(torch.isfinite(V).all().item() and torch.isfinite(J.values).all().item())}
+    if jheights:
+        # the first-order Jacobian at fixed strip heights (lines per workgroup)
+        by = {}
+        for h in jheights:
+            pb.set_jacobian_strip_height(h)
+            by[h] = _time(lambda: pb.jacobian(U, 0.0, J), reps) * 1e6
+        pb.set_jacobian_strip_height(0)
+        r["jacobian_us_by_strip_height"] = by
     log(f"[sedov3d] {label}: {json.dumps(r)}")
     del J, V
     return r
@@ -64,12 +72,15 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--sample", type=float, default=0.05)
     ap.add_argument("--reps", type=int, default=5)
+    ap.add_argument("--jacobian-strip-heights", default="",
+                    help="comma-separated strip heights at which the full mesh's Jacobian is also timed")
     a = ap.parse_args(argv)
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
     t0 = time.perf_counter()
     full = create_structured_mesh((a.n, a.n, a.n), (0.0, 1.2, 0.0, 1.2, 0.0, 1.2), 7)
     log(f"[sedov3d] mesh {a.n}^3 in {time.perf_counter() - t0:.1f} s")
-    out = {"n": a.n, "full": measure(full, "full mesh", a.reps, log)}
+    jh = [int(h) for h in a.jacobian_strip_heights.split(",") if h]
+    out = {"n": a.n, "full": measure(full, "full mesh", a.reps, log, jh)}
     rng = np.random.default_rng(5)
     N = full.sampleMeshSize()
     gids = np.sort(rng.choice(N, size=int(round(a.sample * N)), replace=False))

@@ -73,6 +73,36 @@ def test_jacobian_strip_heights(oracle, torch):
             prob.set_jacobian_strip_height(bad)
 
 
+@pytest.mark.parametrize("shape", [(40, 6, 7), (37, 5, 21), (20, 3, 3)])
+def test_jacobian_3d_strip_heights(shape, oracle, torch):
+    """First-order Sedov 3D: the box's x lines in 16-cell segments (38 cells:
+    16 + 16 + 6; 35: 16 + 16 + 3; 18: 16 + 2), marched in z over strips of 1,
+    2, 3, 8 planes and the grid-size rule (19 box planes: strips of 8 end
+    with a 3-plane strip); pattern bit-exact, values bit-identical across
+    heights and within 1e-12 of the oracle."""
+    case = ("euler3d-sedov-strips-FirstOrder", pda.Euler3d.SedovSymmetry, S.FirstOrder, shape,
+            (0, 1.2, 0, 1.2, 0, 1.2), 3, (), 0, None)
+    mesh, prob = cases.make(case)
+    ms, pb = oracle.from_mesh(mesh), cases.oracle_problem(case)
+    U = cases.perturbed_state(prob, 5)
+    Ud = torch.from_numpy(U).cuda()
+    _, rpo, cio, Jo = oracle.velocity_and_jacobian(ms, pb, U)
+    ref = None
+    for h in (1, 2, 3, 8, 0):
+        print(f"[gpu-strips] 3D jacobian {shape} height {h}", flush=True)
+        prob.set_jacobian_strip_height(h)
+        J = _jac(prob, Ud, torch)
+        if ref is None:
+            ref = J.values.clone()
+            assert np.array_equal(J.rowptr.cpu().numpy(), rpo)
+            assert np.array_equal(J.colidx.cpu().numpy(), cio)
+        else:
+            assert torch.equal(J.values, ref), h
+    err = cases.rel_err(ref.cpu().numpy(), Jo)
+    print(f"[gpu-strips] 3D jacobian {shape}: {err:.2e}", flush=True)
+    assert err <= TOL, err
+
+
 def test_velocity_march_3d_strip_heights(oracle, torch):
     # Sedov 3D, WENO5: 40 grid rows per plane, so 12-row strips end with a
     # 4-row strip; the default rule gives 2 rows here and 12 at 256^3
