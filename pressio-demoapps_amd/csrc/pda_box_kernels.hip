@@ -1006,12 +1006,15 @@ int64_t jacobian_strip_height(const Box &B) {
   return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
 }
 
-// cells per x segment of the first-order 3D Euler Jacobian march
-constexpr int J3SEG = 16;
+// cells per x segment of the first-order 3D Euler Jacobian march: 3 x 21 + 1
+// lanes fill the wavefront (29.4 KB of LDS, 5 waves per CU); at 256^3 and
+// 6-plane strips 5.02 ms per jacobian() against 5.09 with 17 cells and 5.13
+// with 16 (at its best height, 4)
+constexpr int J3SEG = 21;
 
 // default strip height (planes per workgroup) of that march, the fastest
-// of 1..64 at 256^3 (5.09 ms per jacobian() at 4, 5.29 at 2, 5.60 at 8)
-constexpr int64_t J3_STRIP = 4;
+// at 256^3 (5.02 ms per jacobian() at 6, 5.11 at 5, 5.28 at 8, 5.46 at 2)
+constexpr int64_t J3_STRIP = 6;
 
 int launch_jacobian_fo_euler3d_box(const pda_problem_s &p, const double *U, double *J, hipStream_t s) {
   const Box &B = p.box;

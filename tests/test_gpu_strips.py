@@ -75,8 +75,8 @@ def test_jacobian_strip_heights(oracle, torch):
 
 @pytest.mark.parametrize("shape", [(40, 6, 7), (37, 5, 21), (20, 3, 3)])
 def test_jacobian_3d_strip_heights(shape, oracle, torch):
-    """First-order Sedov 3D: the box's x lines in 16-cell segments (38 cells:
-    16 + 16 + 6; 35: 16 + 16 + 3; 18: 16 + 2), marched in z over strips of 1,
+    """First-order Sedov 3D: the box's x lines in 21-cell segments (38 cells:
+    21 + 17; 35: 21 + 14; 18: one partial segment), marched in z over strips of 1,
     2, 3, 8 planes and the grid-size rule (19 box planes: strips of 8 end
     with a 3-plane strip); pattern bit-exact, values bit-identical across
     heights and within 1e-12 of the oracle."""

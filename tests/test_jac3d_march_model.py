@@ -33,7 +33,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, "pressio-demoapps_amd", "csrc", "pda_box_kernels.hip")
-E = 16
+E = 21
 M, RS, PER = 5, 35, 175
 
 
@@ -209,7 +209,7 @@ def run_model(n, rps, bug=None):
     assert (covered[inbox] == 1).all() and (covered[~inbox] == 0).all()
 
 
-@pytest.mark.parametrize("n", [(3, 3, 3), (10, 9, 8), (8, 8, 8), (40, 7, 6), (35, 5, 4), (19, 4, 3)])
+@pytest.mark.parametrize("n", [(3, 3, 3), (10, 9, 8), (8, 8, 8), (40, 7, 6), (45, 5, 4), (24, 4, 3), (66, 3, 5)])
 @pytest.mark.parametrize("rps", [1, 2, 3, 7, 64])
 def test_march_model(n, rps):
     run_model(n, rps)
