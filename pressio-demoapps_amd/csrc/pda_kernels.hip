@@ -197,8 +197,8 @@ __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restric
 }
 
 // ------------------------------- first-order Euler Jacobian, staged rows
-// Inner rows (no ghost, no BC factor; Euler has no source term) with four
-// lanes per row and 16 rows per wavefront.  Lane a < D evaluates the two
+// Rows of the first-order Euler Jacobian (Euler has no source term) with
+// four lanes per row and 16 rows per wavefront.  Lane a < D evaluates the two
 // faces of axis a.  The Euler flux along axis a is the flux along x with
 // momentum components 1 and 1+a exchanged, so every lane runs the x-face
 // code on exchanged states and writes entry (k, j) of its blocks to
@@ -208,12 +208,27 @@ __global__ void __launch_bounds__(64) k_jacobian(View P, const double *__restric
 // generic row code does, and the wavefront then copies each staged row to
 // its contiguous CSR values with coalesced streaming stores (one thread per
 // row would scatter every store over 64 rows); 16 rows consecutive in the
-// CSR leave as one range of 16-byte stores.  Near-boundary rows are left to
-// the generic kernel.  Used where no structured box kernel applies (sample
-// meshes, 2D meshes given as a graph).
+// CSR leave as one range of 16-byte stores.  Used where no structured box
+// kernel applies (sample meshes, 2D meshes given as a graph) and for the
+// boundary rows of a structured 3D full mesh.  Near-boundary rows: a
+// missing neighbour along axis a reads the ghost state sign_k U_k of the row's own cell (the first
+// ghost layer of every ghost rule), writes no block, and adds fac_j JL (left
+// side) or -fac_j JR (right side) of that face, times h, to the axis's
+// diagonal part, fac the row's BC factors (the terms of
+// impl/mixin_directional_flux_balance_jacobian.hpp:312-369, with the factors
+// of impl/euler_3d_prob_class.hpp:626-651).
 constexpr int SJ_LANES = 4, SJ_ROWS = 64 / SJ_LANES;
 
 __device__ __forceinline__ int exch(int k, int a) { return k == 1 ? 1 + a : (k == 1 + a ? 1 : k); }
+
+// ghost sign of native dof exch(k, a) on side `side` of axis a
+template <int D>
+__device__ __forceinline__ double ghost_sign(const View &P, int a, int side, int k) {
+  const double s0 = P.gsign[0][side][exch(k, 0)];
+  const double s1 = D > 1 ? P.gsign[1][side][exch(k, 1)] : 0.;
+  const double s2 = D > 2 ? P.gsign[2][side][exch(k, 2)] : 0.;
+  return a == 0 ? s0 : (a == 1 ? s1 : s2);
+}
 
 template <int D>
 __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double *__restrict__ U,
@@ -227,13 +242,15 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
   int64_t r = -1;
   if (t < P.nvisit) {
     const int64_t rr = P.rows ? P.rows[t] : t;
-    if (P.ghost_row[rr] < 0) r = rr;
+    r = rr;
   }
   if (a == 0) row_of[g] = r;
   const bool act = r >= 0 && a < D;
   double *st = stage + g * PER;
   typename Ph::Side Ln, Lp, Rn, Rp;
   int sL = 0, sR = 0, s0 = 0;
+  bool gl = false, gr_ = false; // no neighbour on the left / right
+  double fac[M];                // BC factors of native dof exch(j, a)
   double h = 0.;
   if (act) {
     int64_t cells[3];
@@ -241,11 +258,14 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
       const int32_t *gr = P.graph + r * P.ncols;
       const uint8_t *sl = P.slot + r * P.npos;
       const int cl = colof<D>(a, 0, 0), cr = colof<D>(a, 1, 0);
-      cells[0] = gr[cl];
+      const int32_t il = gr[cl], ir = gr[cr];
+      gl = il < 0;
+      gr_ = ir < 0;
+      cells[0] = gl ? gr[0] : il;
       cells[1] = gr[0];
-      cells[2] = gr[cr];
-      sL = sl[cl];
-      sR = sl[cr];
+      cells[2] = gr_ ? gr[0] : ir;
+      sL = gl ? 0 : sl[cl];
+      sR = gr_ ? 0 : sl[cr];
       s0 = sl[0];
     }
     double q[3][M];
@@ -253,6 +273,18 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int k = 0; k < M; ++k) q[c][k] = U[cells[c] * M + exch(k, a)];
+#pragma unroll
+    for (int k = 0; k < M; ++k) {
+      if (gl) q[0][k] = ghost_sign<D>(P, a, 0, k) * q[0][k];
+      if (gr_) q[2][k] = ghost_sign<D>(P, a, 1, k) * q[2][k];
+      fac[k] = 0.;
+    }
+    const int64_t gi = P.ghost_row[r];
+    if ((gl || gr_) && gi >= 0) {
+      const double *f = P.bcf + (gi * D + a) * M;
+#pragma unroll
+      for (int k = 0; k < M; ++k) fac[k] = f[exch(k, a)];
+    }
     Ph::template sides<0>(P, q[0], q[1], Ln, Lp);
     Ph::template sides<0>(P, q[1], q[2], Rn, Rp);
     h = P.hinv[a];
@@ -266,8 +298,8 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
 #pragma unroll
       for (int j = 0; j < M; ++j) {
         const int o = exch(k, a) * RS + exch(j, a);
-        st[o + sL * M] = 0. + Ph::template entry<0>(P, Ln, k, j) * h;
-        st[o + sR * M] = 0. + (-Ph::template entry<0>(P, Rp, k, j)) * h;
+        if (!gl) st[o + sL * M] = 0. + Ph::template entry<0>(P, Ln, k, j) * h;
+        if (!gr_) st[o + sR * M] = 0. + (-Ph::template entry<0>(P, Rp, k, j)) * h;
       }
   }
   // every lane forms its axis's part of the diagonal block at once; the
@@ -278,8 +310,11 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
 #pragma unroll
     for (int k = 0; k < M; ++k)
 #pragma unroll
-      for (int j = 0; j < M; ++j)
+      for (int j = 0; j < M; ++j) {
         dg[k][j] = (Ph::template entry<0>(P, Lp, k, j) - Ph::template entry<0>(P, Rn, k, j)) * h;
+        if (gl) dg[k][j] += (fac[j] * Ph::template entry<0>(P, Ln, k, j)) * h;
+        if (gr_) dg[k][j] += (fac[j] * -Ph::template entry<0>(P, Rp, k, j)) * h;
+      }
   }
 #pragma unroll
   for (int ax = 0; ax < D; ++ax) {
@@ -305,6 +340,7 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
   if (run) {
     rb0 = P.rowbase[r0];
     run = P.rowbase[rl] == rb0 + int64_t(SJ_ROWS - 1) * PER;
+    run = run && P.nblk[rl] == NB;
   }
   if (run) {
     constexpr int TOT = SJ_ROWS * PER;
@@ -323,7 +359,14 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler_staged(View P, const double
     if (rg < 0) continue;
     double *dst = J + P.rowbase[rg];
     const double *src = stage + gg * PER;
-    for (int e = lane; e < PER; e += 64) store_stream(dst + e, src[e]);
+    // a near-boundary row holds nblk of the NB block slots per scalar row
+    const int rs = P.nblk[rg] * M;
+    for (int e = lane; e < M * rs; e += 64) {
+      int k = 0;
+#pragma unroll
+      for (int q = 1; q < M; ++q) k += e >= q * rs;
+      store_stream(dst + e, src[k * RS + (e - k * rs)]);
+    }
   }
 }
 
@@ -1221,23 +1264,26 @@ int launch_jacobian(const pda_problem_s &p, const double *U, double, double *J, 
       return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);
     }
     if (!V && !p.view.dup && p.w == 1 && (p.family == PDA_EULER2D || p.family == PDA_EULER3D)) {
-      // first-order Euler Jacobian alone: staged inner rows, then the
-      // near-boundary rows with the generic row code
+      // first-order Euler Jacobian alone
       const View v = generic_view(p, nullptr, 0);
       if (p.family == PDA_EULER3D && p.use_box && p.box.valid && p.box.cells() > 0) {
         // the box's x lines marched in z, then the rows outside the box
-        // with the generic row code
+        // (boundary rows, with ghosts) staged four lanes per row
         const int rc = launch_jacobian_fo_euler3d_box(p, U, J, s);
         if (rc || p.h_rem_rows.empty()) return rc;
-        return jac_generic(generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size())), p, U, V, J, s);
+        const View vr = generic_view(p, &p.d_rem_rows, int64_t(p.h_rem_rows.size()));
+        hipLaunchKernelGGL((k_jac_fo_euler_staged<3>), dim3(unsigned((vr.nvisit + SJ_ROWS - 1) / SJ_ROWS)),
+                           dim3(64), 0, s, vr, U, J);
+        PDA_HIP_CHECK(hipGetLastError());
+        return PDA_OK;
       }
+      // every row, near-boundary rows included, staged four lanes per row
       const unsigned nb = unsigned((v.nvisit + SJ_ROWS - 1) / SJ_ROWS);
       if (p.family == PDA_EULER2D)
         hipLaunchKernelGGL((k_jac_fo_euler_staged<2>), dim3(nb), dim3(64), 0, s, v, U, J);
       else hipLaunchKernelGGL((k_jac_fo_euler_staged<3>), dim3(nb), dim3(64), 0, s, v, U, J);
       PDA_HIP_CHECK(hipGetLastError());
-      if (p.n_near == 0) return PDA_OK;
-      return jac_generic(generic_view(p, &p.d_near_rows, p.n_near), p, U, V, J, s);
+      return PDA_OK;
     }
     return jac_generic(generic_view(p, nullptr, 0), p, U, V, J, s);
   }
