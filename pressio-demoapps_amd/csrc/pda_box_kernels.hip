@@ -707,20 +707,19 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler2d(View P, Box B, int rows_p
 // A wavefront owns a segment of up to E consecutive box cells of an x line
 // (j, k) and marches in z over a strip of planes; consecutive workgroups take
 // consecutive lines j of the same strip, so the waves in flight write
-// neighbouring ranges of the CSR values.  On each line lanes 0..E
-// ("x lanes") evaluate the x faces of the segment (lane l: the face on the
-// left of cell l, lane ncell the face on the right of the last cell), lanes
-// E+1..2E ("y lanes") the two y faces of cell lane-E-1 and lanes 2E+1..3E
-// ("z lanes") the two z faces of cell lane-2E-1; lanes past 3E repeat the
-// last z lane and write nothing.  The 3D Euler flux along axis a is the flux
-// along x with the momentum components 1 and 1+a exchanged, so every lane
-// runs the x-face code on exchanged states and writes entry (r, c) of its
-// blocks to row p(r), column p(c), p that exchange.  For a cell and an axis
-// with faces lo (below) and up (above):
+// neighbouring ranges of the CSR values.  Lane a E + l (a = 0, 1, 2) holds
+// cell l of the segment and evaluates its two faces along axis a (lanes past
+// 3E, and lanes past the segment's last cell, repeat the last cell and write
+// nothing).  The 3D Euler flux along axis a is the flux along x with the
+// momentum components 1 and 1+a exchanged, so every lane runs the x-face
+// code on exchanged states (one instruction stream for the three axes) and
+// writes entry (r, c) of its blocks to row p(r), column p(c), p that
+// exchange.  For a cell and an axis with faces lo (below) and up (above):
 //   block -a = h JL(lo),  block +a = -h JR(up),  diagonal += h (JR(lo) - JL(up)).
-// An x lane forms, from its one face, the -x block of the cell on its right
-// and the +x block of the cell on its left, and takes JL of the next face
-// from the next lane; a y or z lane evaluates both faces of its cell.  The
+// The wave's two face evaluations per lane are the staged kernel's; the x
+// faces inside the segment are evaluated twice (by the cells on both sides,
+// from the same states, so with the same values), which costs no issue
+// slots: the y and z lanes evaluate two faces in the same instructions.  The
 // diagonal adds the axes in LDS in the order x, y, z, as the generic row
 // code does.  The line's blocks are assembled in LDS in CSR order (per
 // cell: scalar row r, then the blocks [-z, -y, -x, self, +x, +y, +z], then
@@ -739,12 +738,11 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler3d(View P, Box B, int rows_p
   using Ph = PhEuler3;
   using Side = Ph::Side;
   constexpr int M = 5, RS = 35, PER = M * RS;
-  static_assert(3 * E + 1 <= 64, "one x lane per face, one y and one z lane per cell");
+  static_assert(3 * E <= 64, "one lane per cell and axis");
   __shared__ double stage[E * PER];
   const int lane = threadIdx.x;
-  const int ax = lane <= E ? 0 : (lane <= 2 * E ? 1 : 2);
-  const int l = ax == 0 ? lane : (ax == 1 ? lane - E - 1 : lane - 2 * E - 1);
-  const bool xl = ax == 0;
+  const int ax = lane < E ? 0 : (lane < 2 * E ? 1 : 2);
+  const int l = lane - ax * E;
   const int64_t nx = B.n[0], nxy = B.n[0] * B.n[1];
   const int64_t bw = B.hi[0] - B.lo[0], bh = B.hi[1] - B.lo[1];
   const int64_t nseg = (bw + E - 1) / E;
@@ -755,29 +753,22 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler3d(View P, Box B, int rows_p
   const int ncell = int(bw - seg * E < E ? bw - seg * E : E);
   const int64_t k0 = B.lo[2] + strip * rows_per_strip;
   const int64_t k1 = k0 + rows_per_strip < B.hi[2] ? k0 + rows_per_strip : B.hi[2];
-  // the lane's cell (x lanes: the cell on the right of the face) and its
-  // states: x lanes read (cell-1, cell), y and z lanes (cell-s, cell, cell+s)
-  const int ci = xl ? (l <= ncell ? l : ncell) : (l < ncell ? l : ncell - 1);
-  const int64_t st = ax == 1 ? nx : nxy;
-  const int64_t sa = xl ? 1 : st, sc = xl ? 0 : st;
+  // the lane's cell and its neighbours along its axis: (cell-s, cell, cell+s)
   const bool own = l < ncell;
+  const int ci = own ? l : ncell - 1;
+  const int64_t st = ax == 0 ? 1 : (ax == 1 ? nx : nxy);
   const double h = P.hinv[ax];
   // row r of an entry goes to row p(r) of the stage, column c to p(c)
   auto pr = [&](int r) { return r == 1 ? 1 + ax : (r == 1 + ax ? 1 : r); };
-  // blocks from the lane's faces: block 1 is the -a block of the lane's
-  // cell, block 2 the +a block of the cell on the left (x lanes) or of the
-  // lane's cell (y and z lanes)
-  const bool v1 = own, v2 = xl ? (l >= 1 && l <= ncell) : own;
-  const int c2 = xl ? l - 1 : l;
-  double *d1 = stage + (v1 ? l : 0) * PER + (2 - ax) * M;
-  double *d2 = stage + (v2 ? c2 : 0) * PER + (4 + ax) * M;
-  double *dd = stage + (own ? l : 0) * PER + 3 * M;
+  double *d1 = stage + (own ? l : 0) * PER + (2 - ax) * M;  // block -a
+  double *d2 = stage + (own ? l : 0) * PER + (4 + ax) * M;  // block +a
+  double *dd = stage + (own ? l : 0) * PER + 3 * M;         // diagonal
   double qa[M], qb[M], qc[M];
   {
     const int64_t c = k0 * nxy + jj * nx + iseg + ci;
-    load_cell<M>(U, c - sa, qa);
+    load_cell<M>(U, c - st, qa);
     load_cell<M>(U, c, qb);
-    load_cell<M>(U, c + sc, qc);
+    load_cell<M>(U, c + st, qc);
   }
   // the first line's states are consumed here, so the loop head waits only
   // for the states prefetched in the previous iteration
@@ -792,9 +783,8 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler3d(View P, Box B, int rows_p
       eb[k] = qb[pr(k)];
       ec[k] = qc[pr(k)];
     }
-    Side A0, A1, B0, B1;
+    Side A0, A1, B0, B1;  // sides of the faces below (A) and above (B) the cell
     Ph::sides<0>(P, ea, eb, A0, A1);
-    // x lanes evaluate a second face from equal states and discard it
     Ph::sides<0>(P, eb, ec, B0, B1);
     {
       // prefetch the next line (the last line reloads itself, so the loads
@@ -802,39 +792,27 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler3d(View P, Box B, int rows_p
       // offset, both arriving while the blocks are formed
       const int64_t kn = kk + 1 < k1 ? kk + 1 : kk;
       const int64_t c = kn * nxy + jj * nx + iseg + ci;
-      load_cell<M>(U, c - sa, qa);
+      load_cell<M>(U, c - st, qa);
       load_cell<M>(U, c, qb);
-      load_cell<M>(U, c + sc, qc);
+      load_cell<M>(U, c + st, qc);
     }
     const int cell0 = __builtin_amdgcn_readfirstlane(int(kk * nxy + jj * nx + iseg));
     const int64_t rb0 = P.rowbase[cell0];
-    Side nA0;
-    {
-      const double *pa = reinterpret_cast<const double *>(&A0);
-      double *pn = reinterpret_cast<double *>(&nA0);
-#pragma unroll
-      for (int q = 0; q < int(sizeof(Side) / sizeof(double)); ++q) pn[q] = lane_next(pa[q]);
-    }
-    const Side Up = pick(xl, nA0, B0);  // JL of the face above the cell
-    const Side X2 = pick(xl, A1, B1);
-    if (v1) {
+    if (own) {
 #pragma unroll
       for (int r = 0; r < M; ++r)
 #pragma unroll
-        for (int c = 0; c < M; ++c) d1[pr(r) * RS + pr(c)] = h * Ph::entry<0>(P, A0, r, c);
-    }
-    if (v2) {
-#pragma unroll
-      for (int r = 0; r < M; ++r)
-#pragma unroll
-        for (int c = 0; c < M; ++c) d2[pr(r) * RS + pr(c)] = -h * Ph::entry<0>(P, X2, r, c);
+        for (int c = 0; c < M; ++c) {
+          d1[pr(r) * RS + pr(c)] = h * Ph::entry<0>(P, A0, r, c);
+          d2[pr(r) * RS + pr(c)] = -h * Ph::entry<0>(P, B1, r, c);
+        }
     }
     double part[M * M];
 #pragma unroll
     for (int r = 0; r < M; ++r)
 #pragma unroll
-      for (int c = 0; c < M; ++c) part[r * M + c] = (Ph::entry<0>(P, A1, r, c) - Ph::entry<0>(P, Up, r, c)) * h;
-    if (own && xl) {
+      for (int c = 0; c < M; ++c) part[r * M + c] = (Ph::entry<0>(P, A1, r, c) - Ph::entry<0>(P, B0, r, c)) * h;
+    if (own && ax == 0) {
 #pragma unroll
       for (int r = 0; r < M; ++r)
 #pragma unroll
@@ -1006,10 +984,10 @@ int64_t jacobian_strip_height(const Box &B) {
   return rps < 1 ? 1 : (rps > 64 ? 64 : rps);
 }
 
-// cells per x segment of the first-order 3D Euler Jacobian march: 3 x 21 + 1
+// cells per x segment of the first-order 3D Euler Jacobian march: 3 x 21
 // lanes fill the wavefront (29.4 KB of LDS, 5 waves per CU); at 256^3 and
 // 6-plane strips 5.02 ms per jacobian() against 5.09 with 17 cells and 5.13
-// with 16 (at its best height, 4)
+// with 16 (at its best height, 4), measured with one lane per x face
 constexpr int J3SEG = 21;
 
 // default strip height (planes per workgroup) of that march, the fastest

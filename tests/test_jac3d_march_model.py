@@ -11,8 +11,7 @@ that
     inside the grid, and each face a lane evaluates joins the two cells the
     blocks it feeds need: the -a block of a cell takes JL of the face below
     it along a, the +a block JR of the face above, the diagonal JR of the
-    face below and JL of the face above (x lanes take the latter from the
-    next lane, which must be an x lane holding that face);
+    face below and JL of the face above;
   - every stage entry of the line's cells is written exactly once by a block
     store or by the x part of the diagonal, and the diagonal's y and z parts
     add to entries the x part wrote;
@@ -43,20 +42,24 @@ def test_model_matches_kernel_source():
     body = txt[txt.index("k_jac_fo_euler3d(View P"):]
     body = body[:body.index("}  // namespace")]
     for line in [
-        "const int ax = lane <= E ? 0 : (lane <= 2 * E ? 1 : 2);",
-        "const int l = ax == 0 ? lane : (ax == 1 ? lane - E - 1 : lane - 2 * E - 1);",
+        "const int ax = lane < E ? 0 : (lane < 2 * E ? 1 : 2);",
+        "const int l = lane - ax * E;",
         "const int64_t seg = bid % nseg, jj = B.lo[1] + (bid / nseg) % bh;",
         "const int64_t strip = bid / (nseg * bh);",
         "const int64_t k1 = k0 + rows_per_strip < B.hi[2] ? k0 + rows_per_strip : B.hi[2];",
         "for (int64_t kk = k0; kk < k1; ++kk) {",
-        "const int ci = xl ? (l <= ncell ? l : ncell) : (l < ncell ? l : ncell - 1);",
-        "const int64_t sa = xl ? 1 : st, sc = xl ? 0 : st;",
-        "const bool v1 = own, v2 = xl ? (l >= 1 && l <= ncell) : own;",
-        "double *d1 = stage + (v1 ? l : 0) * PER + (2 - ax) * M;",
-        "double *d2 = stage + (v2 ? c2 : 0) * PER + (4 + ax) * M;",
-        "double *dd = stage + (own ? l : 0) * PER + 3 * M;",
+        "const int ci = own ? l : ncell - 1;",
+        "const int64_t st = ax == 0 ? 1 : (ax == 1 ? nx : nxy);",
+        "double *d1 = stage + (own ? l : 0) * PER + (2 - ax) * M;  // block -a",
+        "double *d2 = stage + (own ? l : 0) * PER + (4 + ax) * M;  // block +a",
+        "double *dd = stage + (own ? l : 0) * PER + 3 * M;         // diagonal",
+        "load_cell<M>(U, c - st, qa);",
+        "load_cell<M>(U, c + st, qc);",
         "const int64_t kn = kk + 1 < k1 ? kk + 1 : kk;",
         "const int64_t c = kn * nxy + jj * nx + iseg + ci;",
+        "d1[pr(r) * RS + pr(c)] = h * Ph::entry<0>(P, A0, r, c);",
+        "d2[pr(r) * RS + pr(c)] = -h * Ph::entry<0>(P, B1, r, c);",
+        "part[r * M + c] = (Ph::entry<0>(P, A1, r, c) - Ph::entry<0>(P, B0, r, c)) * h;",
         "const int npairs = (total - head) >> 1;",
         "const int q = q0 < npairs ? q0 : npairs - 1;",
         "for (int it = 0; it < (E * PER / 2 + 63) / 64; ++it) {",
@@ -104,24 +107,25 @@ def run_model(n, rps, bug=None):
         k1 = min(k0 + rps, hi[2])
         lanes = []
         for lane in range(64):
-            ax = 0 if lane <= E else (1 if lane <= 2 * E else 2)
-            l = lane if ax == 0 else (lane - E - 1 if ax == 1 else lane - 2 * E - 1)
-            xl = ax == 0
-            ci = (l if l <= ncell else ncell) if xl else (l if l < ncell else ncell - 1)
-            if bug == "unclamped":   # y and z lanes past the last cell keep their own column
-                ci = l if not xl else ci
-            st = nx if ax == 1 else nxy
-            sa, sc = (1, 0) if xl else (st, st)
+            ax = 0 if lane < E else (1 if lane < 2 * E else 2)
+            l = lane - ax * E
             own = l < ncell
-            v1, v2 = own, ((1 <= l <= ncell) if xl else own)
-            c2 = l - 1 if xl else l
-            if bug == "x-right-block-own-cell":
-                c2 = l
-            lanes.append((ax, l, xl, ci, sa, sc, own, v1, v2, c2))
+            ci = l if own else ncell - 1
+            if bug == "unclamped":   # lanes past the last cell keep their own column
+                ci = l
+            xl = ax == 0
+            st = (1, nx, nxy)[ax]
+            sa, sc = st, st
+            v1 = v2 = own
+            c2 = l
+            s2 = 4 + ax
+            if bug == "slot":        # the +a block in the -a slot of the other side
+                s2 = 4 - ax
+            lanes.append((ax, l, xl, ci, sa, sc, own, v1, v2, c2, s2))
         for kk in range(k0, k1):
             for kq in (kk, kk + 1 if kk + 1 < k1 else kk):   # the plane and its prefetch
                 for (ax, l, xl, ci, sa, sc, *_rest) in lanes:
-                    for o in ((-1, 0) if xl else (-1, 0, 1)):
+                    for o in (-1, 0, 1):
                         assert inside(ax, iseg + ci, j, kq, o), (n, rps, bid, kq, l, ax, o)
             line0 = kk * nxy + j * nx + iseg
             # faces each lane evaluates: A = (c - sa, c), B = (c, c + sc)
@@ -132,7 +136,7 @@ def run_model(n, rps, bug=None):
             stride = (1, nx, nxy)
             # stage writes: index -> list of (kind, lane)
             writes = {}
-            for lane, (ax, l, xl, ci, sa, sc, own, v1, v2, c2) in enumerate(lanes):
+            for lane, (ax, l, xl, ci, sa, sc, own, v1, v2, c2, s2) in enumerate(lanes):
                 A, Bf = faces[lane]
                 if v1:
                     cell = line0 + l
@@ -143,24 +147,19 @@ def run_model(n, rps, bug=None):
                             writes.setdefault(idx, []).append(("blk", lane, l, 2 - ax))
                 if v2:
                     cell = line0 + c2
-                    up = A if xl else Bf                      # JR of the face above
-                    assert up == (cell, cell + stride[ax])
+                    assert Bf == (cell, cell + stride[ax])    # JR of the face above
                     for r in range(M):
                         for c in range(M):
-                            idx = c2 * PER + (4 + ax) * M + pr(r, ax) * RS + pr(c, ax)
+                            idx = c2 * PER + s2 * M + pr(r, ax) * RS + pr(c, ax)
                             writes.setdefault(idx, []).append(("blk", lane, c2, 4 + ax))
                 if own:
                     cell = line0 + l
                     assert A == (cell - stride[ax], cell)     # JR of the face below
-                    if xl:
-                        nxt = lanes[lane + 1]
-                        assert nxt[0] == 0 and faces[lane + 1][0] == (cell, cell + 1)
-                    else:
-                        assert Bf == (cell, cell + stride[ax])
+                    assert Bf == (cell, cell + stride[ax])    # JL of the face above
             # diagonal: x stores, then y adds, then z adds
             diag_state = {}
             for phase in range(3):
-                for lane, (ax, l, xl, ci, sa, sc, own, v1, v2, c2) in enumerate(lanes):
+                for lane, (ax, l, xl, ci, sa, sc, own, v1, v2, c2, s2) in enumerate(lanes):
                     if not own or ax != phase:
                         continue
                     for r in range(M):
@@ -215,11 +214,10 @@ def test_march_model(n, rps):
     run_model(n, rps)
 
 
-@pytest.mark.parametrize("bug", ["unclamped", "x-right-block-own-cell"])
+@pytest.mark.parametrize("bug", ["unclamped", "slot"])
 def test_model_flags_planted_mistakes(bug):
-    """The model rejects y and z lanes past the last cell of a partial
-    segment reading their own column (out of the grid at the last segment),
-    and x lanes writing their JR block to their own cell instead of the cell
-    on the left."""
+    """The model rejects lanes past the last cell of a partial segment
+    reading their own column (out of the grid at the last segment), and a
+    +a block stored in the slot of another neighbour."""
     with pytest.raises(AssertionError):
         run_model((19, 4, 3), 2, bug=bug)
