@@ -8,6 +8,9 @@ next row's states only when a strip has two or more rows).  Here:
   - the first-order 2D Euler Jacobian at strip heights 1, 2, 3, 8 and the
     grid-size rule gives bit-identical values, within 1e-12 of the oracle;
   - the 3D velocity march at heights 1, 4, 12 and the rule likewise;
+  - the first-order 3D Euler Jacobian at strip heights 1, 2, 3, 8 and the
+    rule on three shapes, and at 256^3 (the 3D timing's shape) on ~3,000
+    sampled rows against the oracle on a sample mesh of those rows;
   - the 1024 x 1024 Riemann workload itself (WENO5 velocity at the heights
     the step loop picks from, first-order Jacobian at the grid-size rule of
     2 rows): pattern bit-exact, values within 1e-12;
@@ -100,6 +103,50 @@ def test_jacobian_3d_strip_heights(shape, oracle, torch):
             assert torch.equal(J.values, ref), h
     err = cases.rel_err(ref.cpu().numpy(), Jo)
     print(f"[gpu-strips] 3D jacobian {shape}: {err:.2e}", flush=True)
+    assert err <= TOL, err
+
+
+def test_sedov3d_256_jacobian_sampled_rows(oracle, torch):
+    """The 256^3 first-order Sedov Jacobian the 3D timing measures: box x
+    lines in 21-cell segments (the last of 2 cells) marched over 6-plane
+    strips, boundary rows staged with their ghosts.  ~3,000 seeded rows, plus
+    the last segment's cells, cells beside segment, strip and plane edges,
+    and boundary cells, are checked against the oracle on a sample mesh of
+    those rows: same columns (mapped back to full-mesh ids), values within
+    1e-12."""
+    import pyoracle
+    n = 256
+    full = pda.create_structured_mesh((n, n, n), (0.0, 1.2, 0.0, 1.2, 0.0, 1.2), 7)
+    ic = {"sedovRadiusCells": 2.0, "sedovAmbientPressure": 1e-5, "sedovEnergy": 0.851072}
+    prob = pda.create_problem(full, pda.Euler3d.SedovSymmetry, S.FirstOrder, 0, ic)
+    U = cases.perturbed_state(prob, 11)
+    Ud = torch.from_numpy(U).cuda()
+    print("[gpu-strips] 256^3 first-order jacobian", flush=True)
+    J = _jac(prob, Ud, torch)
+    rng = np.random.default_rng(11)
+    pick = set(rng.choice(n ** 3, 3000, replace=False).tolist())
+    for i, j, k in [(253, 1, 1), (254, 1, 1), (254, 254, 254), (253, 100, 6), (253, 100, 7), (1, 1, 1),
+                    (0, 0, 0), (255, 255, 255), (0, 128, 128), (128, 0, 128), (128, 128, 0), (255, 17, 200),
+                    (21, 5, 6), (20, 5, 6), (22, 5, 6), (42, 200, 12), (43, 200, 13), (231, 7, 253)]:
+        pick.add((k * n + j) * n + i)
+    gids = np.array(sorted(pick), dtype=np.int32)
+    g = torch.from_numpy(gids.astype(np.int64)).cuda()
+    starts, ends = J.rowptr[5 * g], J.rowptr[5 * g + 5]
+    lens = ends - starts
+    offs = torch.cumsum(lens, 0) - lens
+    idx = torch.repeat_interleave(starts - offs, lens) + torch.arange(int(lens.sum()), device="cuda")
+    vals = J.values[idx].cpu().numpy()
+    cols = J.colidx[idx].cpu().numpy().astype(np.int64)
+    del J, idx
+    torch.cuda.empty_cache()
+    sm = pda.create_sample_mesh(full, gids)
+    _, stencil = sm.sample_gids()
+    Us = U.reshape(-1, 5)[stencil].ravel()
+    pb = pyoracle.Oracle.problem(3, int(pda.Euler3d.SedovSymmetry), int(S.FirstOrder), [1.4])
+    _, rpo, cio, Jo = oracle.velocity_and_jacobian(oracle.from_mesh(sm), pb, Us)
+    assert np.array_equal(stencil[cio // 5].astype(np.int64) * 5 + cio % 5, cols), "columns differ"
+    err = cases.rel_err(vals, Jo)
+    print(f"[gpu-strips] 256^3 jacobian, {len(gids)} rows: {err:.2e}", flush=True)
     assert err <= TOL, err
 
 
