@@ -113,7 +113,9 @@ def test_sedov3d_256_jacobian_sampled_rows(oracle, torch):
     the last segment's cells, cells beside segment, strip and plane edges,
     and boundary cells, are checked against the oracle on a sample mesh of
     those rows: same columns (mapped back to full-mesh ids), values within
-    1e-12."""
+    1e-12.  The same rows of the WENO5 velocity (the 3D march at 12-row
+    strips and at the rule) likewise; the rows include the blast corner, so
+    that their largest value, which rel_err divides by, is the field's."""
     import pyoracle
     n = 256
     full = pda.create_structured_mesh((n, n, n), (0.0, 1.2, 0.0, 1.2, 0.0, 1.2), 7)
@@ -129,6 +131,13 @@ def test_sedov3d_256_jacobian_sampled_rows(oracle, torch):
                     (0, 0, 0), (255, 255, 255), (0, 128, 128), (128, 0, 128), (128, 128, 0), (255, 17, 200),
                     (21, 5, 6), (20, 5, 6), (22, 5, 6), (42, 200, 12), (43, 200, 13), (231, 7, 253)]:
         pick.add((k * n + j) * n + i)
+    # the blast corner, where |V| and the Jacobian entries peak, so that the
+    # sampled rows' scale is the field's (rel_err divides by the largest
+    # oracle value; rows of cancelling fluxes alone would understate it)
+    for k in range(4):
+        for j in range(4):
+            for i in range(4):
+                pick.add((k * n + j) * n + i)
     gids = np.array(sorted(pick), dtype=np.int32)
     g = torch.from_numpy(gids.astype(np.int64)).cuda()
     starts, ends = J.rowptr[5 * g], J.rowptr[5 * g + 5]
@@ -148,6 +157,22 @@ def test_sedov3d_256_jacobian_sampled_rows(oracle, torch):
     err = cases.rel_err(vals, Jo)
     print(f"[gpu-strips] 256^3 jacobian, {len(gids)} rows: {err:.2e}", flush=True)
     assert err <= TOL, err
+    # the WENO5 velocity of the same state at the march's timed strip height
+    # (12 rows, the autotuner's pick at 256^3) and the rule, on the same rows
+    # (the sample mesh of the 7-point full mesh holds the WENO5 stencils)
+    pw = pda.create_problem(full, pda.Euler3d.SedovSymmetry, S.Weno5, 0, ic)
+    pbw = pyoracle.Oracle.problem(3, int(pda.Euler3d.SedovSymmetry), int(S.Weno5), [1.4])
+    Vo = oracle.velocity(oracle.from_mesh(sm), pbw, Us)
+    for h in (12, 0):
+        pw.set_velocity_strip_height(h)
+        V = pw.createRightHandSide()
+        pw.rightHandSide(Ud, 0.0, V)
+        torch.cuda.synchronize()
+        Vg = V.view(-1, 5)[g].cpu().numpy().ravel()
+        assert np.abs(Vg).max() == float(V.abs().max()), "the sampled rows miss the field's peak"
+        err = cases.rel_err(Vg, Vo)
+        print(f"[gpu-strips] 256^3 WENO5 velocity, height {h}, {len(gids)} rows: {err:.2e}", flush=True)
+        assert err <= TOL, err
 
 
 def test_velocity_march_3d_strip_heights(oracle, torch):
