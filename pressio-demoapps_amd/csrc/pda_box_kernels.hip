@@ -729,8 +729,10 @@ __global__ void __launch_bounds__(64) k_jac_fo_euler2d(View P, Box B, int rows_p
 // two cells inside the grid (no ghost, no BC factor), and its cells' CSR
 // rows hold the seven blocks in that order, consecutive cells consecutive.
 // (Running the rows outside the box in the same launch, as the 2D kernel
-// does, measured slower here: 6.24 vs 5.09 ms per jacobian() at 256^3, the
-// 390k boundary rows' workgroups holding this kernel's registers and LDS.)
+// does, measured slower here: 6.24 vs 5.09 ms per jacobian() at 256^3 with
+// the generic row code first, and 5.16 vs 4.93-4.95 ms with the staged rows
+// interleaved among the box's workgroups, one in seven: their workgroups
+// hold this kernel's registers and LDS.)
 template <int E>
 __global__ void __launch_bounds__(64) k_jac_fo_euler3d(View P, Box B, int rows_per_strip,
                                                         const double *__restrict__ U,
